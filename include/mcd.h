@@ -1,0 +1,232 @@
+/*
+ * mcd.h — C-ABI of the MI355X electron-trajectory transport library (libmcd.so).
+ *
+ * The library replaces the per-electron scattering loop of MC-discrete:
+ *   multiTraj_DS   MC/tools/multiScatter.py:66-116   -> mcd_forward (mode MCD_MODE_DS)
+ *   multiTraj_cont MC/tools/multiScatter.py:171-229  -> mcd_forward (mode MCD_MODE_CONT)
+ *   trajectory_DS  MC/tools/singleScatter.py:10-41   (the device step loop)
+ *   trajectory_cont_cl MC/tools/singleScatter.py:205-239
+ *   probTable.computeBlock MC/tools/probTables.py:415-462 -> mcd_tables_create
+ * The Python side (ctypes) owns every buffer; the library owns only the
+ * opaque handles below.  Every entry point returns an int status (MCD_OK = 0)
+ * and records a message retrievable with mcd_last_error().
+ *
+ * Device pointers passed in are plain HIP device addresses; streams are
+ * hipStream_t passed as void* (NULL = the null stream).  One context is used
+ * by one host thread and one stream at a time: calls on one context that
+ * overlap in time race on its work-queue counters.
+ */
+#ifndef MCD_H
+#define MCD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCD_ABI_VERSION 1
+
+#define MCD_MAX_SHELLS 8
+#define MCD_MAX_CHANNELS (MCD_MAX_SHELLS + 3)
+
+/* status codes */
+enum {
+  MCD_OK = 0,
+  MCD_ERR_INVALID = 1,     /* bad argument or configuration */
+  MCD_ERR_DEVICE = 2,      /* HIP runtime error */
+  MCD_ERR_TABLE = 3,       /* table grid inconsistent with the configuration */
+  MCD_ERR_OVERFLOW = 4,    /* scatter-row buffer full; see mcd_counters.scat_rows */
+  MCD_ERR_NOMEM = 5,
+  MCD_ERR_UNSUPPORTED = 6
+};
+
+/* transport modes (input.file `mode`, `Bethe_model`) */
+enum { MCD_MODE_DS = 0, MCD_MODE_CONT = 1 };
+enum { MCD_BETHE_CLASSICAL = 0, MCD_BETHE_JL = 1, MCD_BETHE_EXPLICIT = 2 };
+
+/* electron outcomes (MC/tools/electron.py:44-84, singleScatter.py:15-23) */
+enum {
+  MCD_OUT_NONE = 0,
+  MCD_OUT_ABS = 1,
+  MCD_OUT_BKS = 2,
+  MCD_OUT_TRSM = 3,
+  MCD_OUT_TRDF = 4,
+  MCD_OUT_LONG = 5
+};
+
+/* scattering channels; label values, not the pick order.  The pick order is
+ * the descending-sigma order with ties kept in insertion order Ruth, Moller,
+ * Gryz<shell> (name_s order), Quinn (MC/tools/scattering.py:39-59,483-508). */
+enum {
+  MCD_CH_RUTH = 0,
+  MCD_CH_MOLLER = 1,
+  MCD_CH_QUINN = 2,
+  MCD_CH_GRYZ0 = 3 /* + shell index in name_s order */
+};
+
+/* per-electron anomaly flags */
+enum {
+  MCD_FLAG_NAN_DIR = 1,    /* binaryCollModel returned NaN (W > (E-E_F)/2) */
+  MCD_FLAG_LOSS_GE_E = 2,  /* energy loss >= energy (reference asserts, scattering.py:605,627,822) */
+  MCD_FLAG_LONG_PATH = 4,  /* continuous-mode path >= 1e4 A (reference asserts, scattering.py:809) */
+  MCD_FLAG_FP64_PICK = 8   /* at least one channel pick was settled by the FP64 guard path */
+};
+
+/* Material constants (MC/tools/material.py:151-302) plus the derived
+ * quantities the host computes once (material.py:117-134). */
+typedef struct mcd_material {
+  int32_t Z;
+  int32_t n_shells;
+  double atwt;       /* g/mol */
+  double density;    /* g/cm^3 */
+  double n_val;      /* valence electrons */
+  double E_val;      /* eV */
+  double bethe_k;    /* Joy-Luo k */
+  double shell_Eb[MCD_MAX_SHELLS]; /* binding energies, eV, name_s order */
+  double shell_n[MCD_MAX_SHELLS];  /* occupancies, name_s order */
+  double atnd;       /* atomic number density, m^-3 */
+  double fermi_e;    /* eV */
+  double plasmon_e;  /* eV */
+} mcd_material;
+
+/* One transport configuration (input.file keys, MC/tools/fileTools.py:8-46). */
+typedef struct mcd_config {
+  mcd_material mat;
+  double E0;         /* eV */
+  double Emin;       /* eV */
+  double Wc;         /* eV, Moller cut-off (DS) */
+  double s_tilt;     /* degrees; dir0 = (-sin t, 0, cos t), multiScatter.py:98-100 */
+  double thickness;  /* Angstrom; +inf = semi-infinite */
+  int32_t mode;      /* MCD_MODE_* */
+  int32_t bethe_model; /* MCD_BETHE_* */
+  int64_t max_scatt; /* <= 0: no limit */
+  double J;          /* mean ionisation energy for the Bethe laws, eV (stoppingPowers.py:134-137) */
+  double explicit_lnEstar; /* explicit Bethe: sum_i n_i ln E_i / Z_eff (host-computed, FP64) */
+  double explicit_Zeff;    /* explicit Bethe: n_val + sum_i n_i */
+} mcd_config;
+
+/* Inverse-CDF energy-loss table grid (probTables.py:70-535).  Es has nE
+ * float32 energies, Ws has nW float32 losses; the CDF is nE x (nW-1) float32,
+ * row-major, entries with W > W_max(E) stored as 0 (probTables.py:500-507). */
+typedef struct mcd_table_grid {
+  int32_t nE;
+  int32_t nW;
+  const float* Es;
+  const float* Ws;
+  double Wmin;       /* lower integration limit: Wc (Moller) or E_b (Gryzinski) */
+  double n_el;       /* electrons in the shell: n_val (Moller) or ns (Gryzinski) */
+  double Eb;         /* Gryzinski binding energy; 0 for Moller */
+  int32_t kind;      /* 0 Moller, 1 Gryzinski */
+} mcd_table_grid;
+
+enum { MCD_TABLE_MOLLER = 0, MCD_TABLE_GRYZ = 1 };
+enum { MCD_PREC_FP64 = 0, MCD_PREC_FP32 = 1 };
+
+/* Per-electron record written by mcd_forward (the `els` row(s) of one electron,
+ * electron.py:105-113).  outcome2 != 0 marks the second saveOutcomes row that
+ * the reference appends when an exit step also absorbs (electron.py:52-54). */
+typedef struct mcd_electron_rec {
+  float final_E;
+  float dir[3];
+  float pos[3];
+  uint32_t n_scatt;  /* scatter events (loop iterations that reached det_type) */
+  uint8_t outcome;
+  uint8_t outcome2;
+  uint8_t flags;
+  uint8_t pad;
+} mcd_electron_rec;
+
+/* One `scats` row (scattering.py:527-679, electron.py:59,99), appended to a
+ * device buffer in arbitrary order; sort by (config, electron, step). */
+typedef struct mcd_scatter_rec {
+  uint32_t electron;
+  uint32_t step;
+  uint16_t config;
+  uint8_t channel;   /* MCD_CH_* */
+  uint8_t outcome;   /* outcome set during this step, 0 if none */
+  float E;           /* pre-step energy */
+  float E_loss;
+  float pol;         /* cos^2(theta/2) */
+  float az;          /* halfPhi */
+  float pathl;
+  float x, y, z;     /* pre-step position */
+  float energy;      /* post-step energy */
+  float dx, dy, dz;  /* post-step direction */
+  uint32_t step_flags; /* MCD_FLAG_* raised during this step */
+} mcd_scatter_rec;
+
+/* Device-side append buffer for scatter rows. */
+typedef struct mcd_scatter_buf {
+  mcd_scatter_rec* rows;     /* device */
+  uint64_t capacity;         /* rows */
+  uint64_t max_electron;     /* record only electrons with id < max_electron */
+} mcd_scatter_buf;
+
+/* Counters written by mcd_forward into caller-owned device memory. */
+enum {
+  MCD_CNT_EVENTS = 0,     /* scatter events */
+  MCD_CNT_ELECTRONS = 1,  /* electrons finished */
+  MCD_CNT_BKS = 2,
+  MCD_CNT_ABS = 3,
+  MCD_CNT_TRSM = 4,
+  MCD_CNT_LONG = 5,
+  MCD_CNT_FP64_PICKS = 6, /* picks settled by the FP64 guard */
+  MCD_CNT_SCAT_ROWS = 7,  /* scatter rows requested (may exceed capacity) */
+  MCD_CNT_QUEUE = 8,      /* internal: persistent-kernel work counter */
+  MCD_NCOUNTERS = 16
+};
+
+/* Launch tuning (0 = library default). */
+typedef struct mcd_launch {
+  int32_t block;        /* threads per workgroup */
+  int32_t waves_per_simd; /* resident waves per SIMD for the persistent grid */
+  int32_t precision;    /* MCD_PREC_FP32 transport (default) or MCD_PREC_FP64 (all-FP64 run) */
+  int32_t refill_period;/* steps between wavefront refills (1 = every step) */
+} mcd_launch;
+
+typedef struct mcd_ctx mcd_ctx;
+typedef struct mcd_tables mcd_tables;
+
+/* library version */
+int mcd_version(void);
+
+/* context */
+int mcd_ctx_create(int device, mcd_ctx** out);
+int mcd_ctx_destroy(mcd_ctx* ctx);
+/* copies the last error message of ctx (or of the library if ctx is NULL) */
+int mcd_last_error(const mcd_ctx* ctx, char* buf, size_t len);
+
+/* Build the inverse-CDF tables on the device from host grids (FP64 or FP32
+ * arithmetic), replacing probTable.computeBlock (probTables.py:415-462).
+ * gryz may be NULL for continuous mode.  The grids are copied. */
+int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table_grid* moller,
+                      const mcd_table_grid* gryz, int precision, mcd_tables** out);
+/* As mcd_tables_create, then replace the device CDFs with host CDFs in the
+ * reference layout (nE x (nW-1), masked = 0), the tables the reference reads
+ * back from its memmap files (probTable.readFromMemory, probTables.py:513-535).
+ * A NULL cdf keeps the device-built table. */
+int mcd_tables_upload(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table_grid* moller, const float* cdf_moller,
+                      const mcd_table_grid* gryz, const float* cdf_gryz, mcd_tables** out);
+/* Copy one table's CDF back to the host in the reference layout. */
+int mcd_tables_download(mcd_ctx* ctx, const mcd_tables* t, int which, float* cdf_host, size_t n_floats);
+int mcd_tables_destroy(mcd_tables* t);
+/* Channel-pick table diagnostics (no reference counterpart): node count, cells
+ * with a cross-section crossing, cells always settled in FP64, guard band. */
+int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap_cells, int* n_exact_cells, float* guard_delta);
+
+/* Run electrons [first, first+n) of one configuration.  els: device array of
+ * n records (index = electron - first).  scats may be NULL.  counters: device
+ * array of MCD_NCOUNTERS uint64, accumulated into (not cleared).  Philox key =
+ * seed, counter = (electron, config_id, step, 0).  Asynchronous on stream. */
+int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
+                const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCD_H */

@@ -1,0 +1,492 @@
+// mcd_api.hip — the C-ABI of libmcd.so (declared in include/mcd.h).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "mcd_device.h"
+#include "mcd_internal.h"
+
+using namespace mcd;
+
+struct mcd_ctx {
+  int device;
+  int n_cu;
+  unsigned long long* d_queue;
+  char err[512];
+};
+
+struct TableDev {
+  float* Es = nullptr;
+  float* Ws = nullptr;
+  float* cdf = nullptr;
+  int nE = 0, nW = 0;
+  float E_first = 0.f, inv_h = 0.f;
+};
+
+struct mcd_tables {
+  int device;
+  mcd_config cfg;
+  bool has_w;
+  TableDev m, g;
+  float* nodes;
+  int n_nodes, stride, n_ch;
+  double lnE0, dln;
+  float delta;
+  int n_swap_cells, n_exact_cells;
+};
+
+static thread_local char g_err[512];
+
+static int set_err(mcd_ctx* ctx, int code, const char* fmt, ...) {
+  char* buf = ctx ? ctx->err : g_err;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                        \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess) return set_err(ctx, MCD_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+extern "C" int mcd_version(void) { return MCD_ABI_VERSION; }
+
+extern "C" int mcd_last_error(const mcd_ctx* ctx, char* buf, size_t len) {
+  if (!buf || !len) return MCD_ERR_INVALID;
+  snprintf(buf, len, "%s", ctx ? ctx->err : g_err);
+  return MCD_OK;
+}
+
+extern "C" int mcd_ctx_create(int device, mcd_ctx** out) {
+  if (!out) return set_err(nullptr, MCD_ERR_INVALID, "mcd_ctx_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(nullptr, hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return set_err(nullptr, MCD_ERR_INVALID, "mcd_ctx_create: device %d not present (%d devices)", device, ndev);
+  HIP_TRY(nullptr, hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(nullptr, hipGetDeviceProperties(&prop, device));
+  mcd_ctx* c = new mcd_ctx();
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  c->err[0] = 0;
+  if (hipMalloc(&c->d_queue, 64 * sizeof(unsigned long long)) != hipSuccess) {
+    delete c;
+    return set_err(nullptr, MCD_ERR_NOMEM, "mcd_ctx_create: hipMalloc of the work counters failed");
+  }
+  *out = c;
+  return MCD_OK;
+}
+
+extern "C" int mcd_ctx_destroy(mcd_ctx* ctx) {
+  if (!ctx) return MCD_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipFree(ctx->d_queue);
+  delete ctx;
+  return MCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+static void pair_of(double v, float& hi, float& lo) {
+  hi = (float)v;
+  lo = (float)(v - (double)hi);
+}
+
+static int check_config(mcd_ctx* ctx, const mcd_config* cfg) {
+  if (!cfg) return set_err(ctx, MCD_ERR_INVALID, "config is NULL");
+  const mcd_material& m = cfg->mat;
+  if (m.n_shells < 1 || m.n_shells > MCD_MAX_SHELLS)
+    return set_err(ctx, MCD_ERR_INVALID, "n_shells %d outside [1, %d]", m.n_shells, MCD_MAX_SHELLS);
+  if (!(cfg->E0 > cfg->Emin) || !(cfg->Emin > 0.))
+    return set_err(ctx, MCD_ERR_INVALID, "need E0 > Emin > 0 (E0=%g, Emin=%g)", cfg->E0, cfg->Emin);
+  if (!(m.atnd > 0.) || !(m.fermi_e > 0.) || !(m.plasmon_e > 0.))
+    return set_err(ctx, MCD_ERR_INVALID, "material derived constants (atnd, E_F, E_pl) must be positive");
+  if (cfg->mode != MCD_MODE_DS && cfg->mode != MCD_MODE_CONT)
+    return set_err(ctx, MCD_ERR_INVALID, "unknown mode %d", cfg->mode);
+  if (cfg->mode == MCD_MODE_DS && !(cfg->Wc > 0.))
+    return set_err(ctx, MCD_ERR_INVALID, "DS mode needs Wc > 0");
+  if (cfg->mode == MCD_MODE_CONT && (cfg->bethe_model < 0 || cfg->bethe_model > 2))
+    return set_err(ctx, MCD_ERR_INVALID, "unknown Bethe model %d", cfg->bethe_model);
+  if (!(cfg->thickness > 0.)) return set_err(ctx, MCD_ERR_INVALID, "thickness must be > 0");
+  if (cfg->mode == MCD_MODE_CONT) {
+    // the Bethe laws must remove energy down to Emin, or a trajectory never ends
+    const double E = cfg->Emin, J = cfg->J;
+    double arg = cfg->bethe_model == MCD_BETHE_CLASSICAL ? 1.166 * E / J
+                 : cfg->bethe_model == MCD_BETHE_JL        ? 1.166 * (E + m.bethe_k * J) / J
+                                                           : exp(log(E) - cfg->explicit_lnEstar);
+    if (!(J > 0.) || !(arg > 1.))
+      return set_err(ctx, MCD_ERR_INVALID, "stopping power is not positive at Emin=%g eV for this Bethe model", E);
+  }
+  return MCD_OK;
+}
+
+static DevCfg make_devcfg(const mcd_config& cfg, const mcd_tables* t, uint32_t config_id) {
+  DevCfg c;
+  memset(&c, 0, sizeof(c));
+  const mcd_material& m = cfg.mat;
+  c.mode = cfg.mode;
+  c.bethe = cfg.bethe_model;
+  c.n_shells = m.n_shells;
+  c.Z = m.Z;
+  c.config_id = config_id;
+  // maxScatt off: a cap of 2^24 steps keeps every wavefront finite (a
+  // trajectory is ~10^3 steps; the cap only binds on a runaway random walk)
+  c.max_scatt = cfg.max_scatt > 0 ? cfg.max_scatt : (int64_t)kMaxStepsOff;
+  c.E0 = cfg.E0; c.Emin = cfg.Emin; c.Wc = cfg.Wc;
+  c.Ef = m.fermi_e; c.Epl = m.plasmon_e; c.atnd = m.atnd;
+  c.thickness = cfg.thickness;
+  c.c_alpha = 3.4 * pow((double)m.Z, 0.666667);       // crossSections.py:27
+  c.ZZ = (double)(m.Z * m.Z);
+  c.n_val = m.n_val;
+  c.J = cfg.J; c.bethe_kk = m.bethe_k;
+  c.zeff = cfg.explicit_Zeff; c.lnEstar = cfg.explicit_lnEstar; c.E_val = m.E_val;
+  for (int i = 0; i < MCD_MAX_SHELLS; ++i) { c.Eb[i] = m.shell_Eb[i]; c.ns[i] = m.shell_n[i]; }
+  const double tr = cfg.s_tilt * (3.141592653589793 / 180.);  // np.radians
+  c.dir0[0] = -sin(tr); c.dir0[1] = 0.; c.dir0[2] = cos(tr);
+  pair_of(cfg.E0, c.E0_hi, c.E0_lo);
+  pair_of(cfg.Emin, c.Emin_hi, c.Emin_lo);
+  pair_of(m.fermi_e, c.Ef2_hi, c.Ef2_lo);
+  pair_of(m.plasmon_e, c.Epl_hi, c.Epl_lo);
+  c.thickness_f = (float)cfg.thickness;
+  c.c_alpha_f = (float)c.c_alpha;
+  c.ruth_k_f = (float)(5.21e-21 * c.ZZ * 4. * 3.141592653589793);
+  c.mfp_k_f = (float)(1e14 / m.atnd);
+  c.bethe_k1_f = (float)(2. * kPiEFour * m.atnd * (double)m.Z * 1e-14);
+  c.bethe_c1_f = (float)(1.166 / cfg.J);
+  c.bethe_kJ_f = (float)(m.bethe_k * cfg.J);
+  c.expl_k_f = (float)(2. * kPiEFour * m.atnd * 1e-14 * cfg.explicit_Zeff);
+  c.expl_inv_Estar_f = (float)exp(-cfg.explicit_lnEstar);
+  for (int i = 0; i < 3; ++i) c.dir0_f[i] = (float)c.dir0[i];
+  if (t && t->has_w) {
+    const TableDev* src[2] = {&t->m, &t->g};
+    DevTable* dst[2] = {&c.tm, &c.tg};
+    for (int q = 0; q < 2; ++q) {
+      dst[q]->Es = src[q]->Es; dst[q]->Ws = src[q]->Ws; dst[q]->cdf = src[q]->cdf;
+      dst[q]->nE = src[q]->nE; dst[q]->nW = src[q]->nW;
+      dst[q]->E_first = src[q]->E_first; dst[q]->inv_h = src[q]->inv_h;
+    }
+  }
+  if (t && t->nodes) {
+    c.pick.nodes = t->nodes;
+    c.pick.n_nodes = t->n_nodes;
+    c.pick.stride = t->stride;
+    c.pick.n_ch = t->n_ch;
+    c.pick.lnE0 = (float)t->lnE0;
+    c.pick.inv_dlnE = (float)(1.0 / t->dln);
+    c.pick.delta = t->delta;
+  }
+  return c;
+}
+
+static void free_table(TableDev& t) {
+  if (t.Es) (void)hipFree(t.Es);
+  if (t.Ws) (void)hipFree(t.Ws);
+  if (t.cdf) (void)hipFree(t.cdf);
+  t = TableDev();
+}
+
+extern "C" int mcd_tables_destroy(mcd_tables* t) {
+  if (!t) return MCD_OK;
+  (void)hipSetDevice(t->device);
+  free_table(t->m);
+  free_table(t->g);
+  if (t->nodes) (void)hipFree(t->nodes);
+  delete t;
+  return MCD_OK;
+}
+
+static int upload_grid(mcd_ctx* ctx, const mcd_table_grid* gr, TableDev& td) {
+  if (!gr || !gr->Es || !gr->Ws || gr->nE < 2 || gr->nW < 2)
+    return set_err(ctx, MCD_ERR_TABLE, "table grid missing or smaller than 2 x 2");
+  for (int i = 1; i < gr->nE; ++i)
+    if (!(gr->Es[i] > gr->Es[i - 1])) return set_err(ctx, MCD_ERR_TABLE, "table energies not increasing at %d", i);
+  for (int i = 1; i < gr->nW; ++i)
+    if (!(gr->Ws[i] > gr->Ws[i - 1])) return set_err(ctx, MCD_ERR_TABLE, "table losses not increasing at %d", i);
+  td.nE = gr->nE;
+  td.nW = gr->nW;
+  td.E_first = gr->Es[0];
+  td.inv_h = (float)((double)(gr->nE - 1) / ((double)gr->Es[gr->nE - 1] - (double)gr->Es[0]));
+  const size_t ncdf = (size_t)gr->nE * (size_t)(gr->nW - 1);
+  if (hipMalloc(&td.Es, gr->nE * sizeof(float)) != hipSuccess || hipMalloc(&td.Ws, gr->nW * sizeof(float)) != hipSuccess ||
+      hipMalloc(&td.cdf, ncdf * sizeof(float)) != hipSuccess)
+    return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for a %d x %d table failed", gr->nE, gr->nW - 1);
+  HIP_TRY(ctx, hipMemcpy(td.Es, gr->Es, gr->nE * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(td.Ws, gr->Ws, gr->nW * sizeof(float), hipMemcpyHostToDevice));
+  return MCD_OK;
+}
+
+static int pick_nodes_env() {
+  const char* s = getenv("MCD_PICK_NODES");
+  int n = s ? atoi(s) : 512;
+  return n < 16 ? 16 : (n > 8192 ? 8192 : n);
+}
+
+static uint32_t label_at(const float* node, int n_ch, int pos) {
+  uint32_t lo, hi;
+  memcpy(&lo, &node[n_ch], 4);
+  memcpy(&hi, &node[n_ch + 1], 4);
+  return pos < 8 ? (lo >> (4 * pos)) & 15u : (hi >> (4 * (pos - 8))) & 15u;
+}
+
+static void set_meta(float* node, int n_ch, uint32_t swap, bool exact) {
+  uint32_t hi;
+  memcpy(&hi, &node[n_ch + 1], 4);
+  hi = (hi & 0xFFFu) | (swap << 12) | (exact ? kMetaExact : 0u);
+  memcpy(&node[n_ch + 1], &hi, 4);
+}
+
+// Build the LDS pick table for a DS configuration (see mcd_device.h).
+static int build_pick_table(mcd_ctx* ctx, mcd_tables* t) {
+  const mcd_config& cfg = t->cfg;
+  t->n_ch = cfg.mat.n_shells + 3;
+  t->stride = ((t->n_ch + 2) + 3) / 4 * 4;
+  t->n_nodes = pick_nodes_env();
+  t->lnE0 = log(cfg.Emin);
+  t->dln = (log(cfg.E0) - t->lnE0) / (double)(t->n_nodes - 1);
+  t->delta = 0.f;
+  const size_t nf = (size_t)t->n_nodes * t->stride;
+  if (hipMalloc(&t->nodes, nf * sizeof(float)) != hipSuccess)
+    return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the pick table failed");
+  DevCfg c = make_devcfg(cfg, t, 0);
+  HIP_TRY(ctx, launch_pick_nodes(c, t->n_nodes, t->lnE0, t->dln, t->nodes, t->stride, 0));
+  std::vector<float> h(nf);
+  HIP_TRY(ctx, hipMemcpy(h.data(), t->nodes, nf * sizeof(float), hipMemcpyDeviceToHost));
+  // cell classification from the node orders
+  t->n_swap_cells = t->n_exact_cells = 0;
+  std::vector<char> exact(t->n_nodes, 0);
+  for (int k = 0; k + 1 < t->n_nodes; ++k) {
+    float* a = &h[(size_t)k * t->stride];
+    float* b = &h[(size_t)(k + 1) * t->stride];
+    int ndiff = 0, first = -1;
+    for (int q = 0; q < t->n_ch; ++q)
+      if (label_at(a, t->n_ch, q) != label_at(b, t->n_ch, q)) { if (first < 0) first = q; ++ndiff; }
+    uint32_t swap = kMetaNoSwap;
+    bool ex = false;
+    if (ndiff == 2 && first + 1 < t->n_ch && label_at(a, t->n_ch, first) == label_at(b, t->n_ch, first + 1) &&
+        label_at(a, t->n_ch, first + 1) == label_at(b, t->n_ch, first))
+      swap = (uint32_t)first;
+    else if (ndiff != 0)
+      ex = true;
+    for (int q = 0; q < t->stride; ++q)
+      if (!(h[(size_t)k * t->stride + q] == h[(size_t)k * t->stride + q])) ex = true;  // NaN node
+    if (swap != kMetaNoSwap) t->n_swap_cells++;
+    exact[k] = ex;
+    set_meta(a, t->n_ch, swap, ex);
+  }
+  HIP_TRY(ctx, hipMemcpy(t->nodes, h.data(), nf * sizeof(float), hipMemcpyHostToDevice));
+  // interpolation check against the exact FP64 pick
+  const int M = 16;
+  const size_t npts = (size_t)(t->n_nodes - 1) * M;
+  float* d_err = nullptr;
+  int* d_bad = nullptr;
+  if (hipMalloc(&d_err, 2 * npts * sizeof(float)) != hipSuccess || hipMalloc(&d_bad, npts * sizeof(int)) != hipSuccess)
+    return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the pick-table check failed");
+  c = make_devcfg(cfg, t, 0);
+  hipError_t e1 = launch_pick_check(c, M, d_err, d_bad, 0);
+  std::vector<float> herr(2 * npts);
+  std::vector<int> hbad(npts);
+  hipError_t e2 = hipMemcpy(herr.data(), d_err, 2 * npts * sizeof(float), hipMemcpyDeviceToHost);
+  hipError_t e3 = hipMemcpy(hbad.data(), d_bad, npts * sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(d_err);
+  (void)hipFree(d_bad);
+  HIP_TRY(ctx, e1);
+  HIP_TRY(ctx, e2);
+  HIP_TRY(ctx, e3);
+  double maxerr = 0.;
+  for (int k = 0; k + 1 < t->n_nodes; ++k) {
+    bool bad = exact[k];
+    double ek = 0.;
+    for (int m = 0; m < M; ++m) {
+      const size_t i = (size_t)k * M + m;
+      if (hbad[i]) bad = true;
+      ek = fmax(ek, fmax((double)herr[2 * i], (double)herr[2 * i + 1]));
+      if (!(herr[2 * i] == herr[2 * i]) || !(herr[2 * i + 1] == herr[2 * i + 1])) bad = true;
+    }
+    if (ek > 1e-4) bad = true;  // badly resolved cell: always exact
+    if (bad) {
+      if (!exact[k]) {
+        uint32_t hi;
+        memcpy(&hi, &h[(size_t)k * t->stride + t->n_ch + 1], 4);
+        set_meta(&h[(size_t)k * t->stride], t->n_ch, (hi >> 12) & 15u, true);
+      }
+      exact[k] = 1;
+      t->n_exact_cells++;
+    } else {
+      maxerr = fmax(maxerr, ek);
+    }
+  }
+  t->delta = (float)(2.0 * maxerr + 2e-7);
+  HIP_TRY(ctx, hipMemcpy(t->nodes, h.data(), nf * sizeof(float), hipMemcpyHostToDevice));
+  return MCD_OK;
+}
+
+extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table_grid* moller,
+                                 const mcd_table_grid* gryz, int precision, mcd_tables** out) {
+  if (!ctx || !out) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_create: NULL argument");
+  *out = nullptr;
+  int rc = check_config(ctx, cfg);
+  if (rc) return rc;
+  if (precision != MCD_PREC_FP64 && precision != MCD_PREC_FP32)
+    return set_err(ctx, MCD_ERR_INVALID, "unknown table precision %d", precision);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  mcd_tables* t = new mcd_tables();
+  memset(t, 0, sizeof(*t));
+  t->device = ctx->device;
+  t->cfg = *cfg;
+  if (cfg->mode == MCD_MODE_DS) {
+    if (!moller || !gryz) {
+      delete t;
+      return set_err(ctx, MCD_ERR_TABLE, "DS mode needs the Moller and Gryzinski table grids");
+    }
+    if ((rc = upload_grid(ctx, moller, t->m)) || (rc = upload_grid(ctx, gryz, t->g))) {
+      mcd_tables_destroy(t);
+      return rc;
+    }
+    t->has_w = true;
+    const mcd_table_grid* gr[2] = {moller, gryz};
+    TableDev* td[2] = {&t->m, &t->g};
+    for (int q = 0; q < 2; ++q) {
+      hipError_t e = launch_build_cdf(td[q]->Es, td[q]->Ws, td[q]->nE, td[q]->nW, cfg->mat.fermi_e, gr[q]->kind,
+                                      gr[q]->n_el, gr[q]->Eb, precision, td[q]->cdf, 0);
+      if (e != hipSuccess) {
+        mcd_tables_destroy(t);
+        return set_err(ctx, MCD_ERR_DEVICE, "CDF build: %s", hipGetErrorString(e));
+      }
+    }
+    if ((rc = build_pick_table(ctx, t))) {
+      mcd_tables_destroy(t);
+      return rc;
+    }
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      mcd_tables_destroy(t);
+      return set_err(ctx, MCD_ERR_DEVICE, "table build: %s", hipGetErrorString(e));
+    }
+  }
+  *out = t;
+  return MCD_OK;
+}
+
+// Same as mcd_tables_create, then replace the device CDFs with host CDFs in the
+// reference layout (masked = 0; probTable.readFromMemory, probTables.py:513-535).
+extern "C" int mcd_tables_upload(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table_grid* moller,
+                                     const float* cdf_moller, const mcd_table_grid* gryz, const float* cdf_gryz,
+                                     mcd_tables** out) {
+  int rc = mcd_tables_create(ctx, cfg, moller, gryz, MCD_PREC_FP64, out);
+  if (rc) return rc;
+  mcd_tables* t = *out;
+  if (!t->has_w) return MCD_OK;
+  const float* src[2] = {cdf_moller, cdf_gryz};
+  TableDev* td[2] = {&t->m, &t->g};
+  for (int q = 0; q < 2; ++q) {
+    if (!src[q]) continue;
+    const size_t n = (size_t)td[q]->nE * (size_t)(td[q]->nW - 1);
+    float* tmp = nullptr;
+    if (hipMalloc(&tmp, n * sizeof(float)) != hipSuccess) {
+      mcd_tables_destroy(t);
+      *out = nullptr;
+      return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the CDF upload failed");
+    }
+    hipError_t e = hipMemcpy(tmp, src[q], n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_mask(tmp, td[q]->cdf, n, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(tmp);
+    if (e != hipSuccess) {
+      mcd_tables_destroy(t);
+      *out = nullptr;
+      return set_err(ctx, MCD_ERR_DEVICE, "CDF upload: %s", hipGetErrorString(e));
+    }
+  }
+  return MCD_OK;
+}
+
+extern "C" int mcd_tables_download(mcd_ctx* ctx, const mcd_tables* t, int which, float* cdf_host, size_t n_floats) {
+  if (!ctx || !t || !cdf_host) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_download: NULL argument");
+  if (!t->has_w) return set_err(ctx, MCD_ERR_INVALID, "tables hold no energy-loss tables (continuous mode)");
+  const TableDev& td = which == MCD_TABLE_MOLLER ? t->m : t->g;
+  const size_t n = (size_t)td.nE * (size_t)(td.nW - 1);
+  if (n_floats < n) return set_err(ctx, MCD_ERR_INVALID, "buffer holds %zu floats, table has %zu", n_floats, n);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  float* tmp = nullptr;
+  if (hipMalloc(&tmp, n * sizeof(float)) != hipSuccess) return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc failed");
+  hipError_t e = launch_unmask(td.cdf, tmp, n, 0);
+  if (e == hipSuccess) e = hipMemcpy(cdf_host, tmp, n * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  HIP_TRY(ctx, e);
+  return MCD_OK;
+}
+
+// Pick-table diagnostics: n_nodes, stride, swap cells, exact cells, delta.
+extern "C" int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap, int* n_exact, float* delta) {
+  if (!t) return MCD_ERR_INVALID;
+  if (n_nodes) *n_nodes = t->n_nodes;
+  if (n_swap) *n_swap = t->n_swap_cells;
+  if (n_exact) *n_exact = t->n_exact_cells;
+  if (delta) *delta = t->delta;
+  return MCD_OK;
+}
+
+static bool same_config(const mcd_config& a, const mcd_config& b) {
+  return a.mat.Z == b.mat.Z && a.mat.atnd == b.mat.atnd && a.mat.fermi_e == b.mat.fermi_e && a.E0 == b.E0 &&
+         a.Emin == b.Emin && a.Wc == b.Wc && a.mat.n_shells == b.mat.n_shells;
+}
+
+extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                           uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
+                           const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch, void* stream) {
+  if (!ctx) return set_err(nullptr, MCD_ERR_INVALID, "mcd_forward: ctx is NULL");
+  int rc = check_config(ctx, cfg);
+  if (rc) return rc;
+  if (!counters) return set_err(ctx, MCD_ERR_INVALID, "mcd_forward: counters is NULL");
+  if (n == 0) return MCD_OK;
+  if (!els) return set_err(ctx, MCD_ERR_INVALID, "mcd_forward: els is NULL");
+  if (config_id > 0xFFFFu) return set_err(ctx, MCD_ERR_INVALID, "config_id must fit 16 bits");
+  const int precision = launch ? launch->precision : MCD_PREC_FP32;
+  if (cfg->mode == MCD_MODE_DS) {
+    if (!tables || !tables->has_w) return set_err(ctx, MCD_ERR_TABLE, "DS mode needs tables from mcd_tables_create");
+    if (!same_config(*cfg, tables->cfg))
+      return set_err(ctx, MCD_ERR_TABLE, "tables were built for a different configuration");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  DevCfg c = make_devcfg(*cfg, tables, config_id);
+  RunArgs R;
+  memset(&R, 0, sizeof(R));
+  R.seed = seed;
+  R.first = first;
+  R.n = n;
+  R.els = els;
+  R.counters = (unsigned long long*)counters;
+  R.queue = ctx->d_queue;
+  if (scats && scats->rows && scats->capacity) {
+    R.rows = scats->rows;
+    R.row_cap = scats->capacity;
+    R.row_max_electron = scats->max_electron;
+    R.row_cursor = (unsigned long long*)&counters[MCD_CNT_SCAT_ROWS];
+  }
+  int block = launch && launch->block > 0 ? launch->block : (precision == MCD_PREC_FP64 ? 256 : 256);
+  if (block % 64 || block > (precision == MCD_PREC_FP64 ? 256 : 512))
+    return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64 and at most %d", block,
+                   precision == MCD_PREC_FP64 ? 256 : 512);
+  int per_cu = max_blocks_per_cu(c, precision, block);
+  if (launch && launch->waves_per_simd > 0) {
+    const int cap = launch->waves_per_simd * 4 / (block / 64);
+    if (cap >= 1 && cap < per_cu) per_cu = cap;
+  }
+  uint64_t want = (n + block - 1) / block;
+  uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)per_cu;
+  if (grid > want) grid = want;
+  if (grid < 1) grid = 1;
+  HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
+  HIP_TRY(ctx, launch_transport(c, R, precision, (int)grid, block, s));
+  return MCD_OK;
+}

@@ -1,0 +1,366 @@
+// mcd_device.h — device-side building blocks of the gfx950 transport kernels.
+//
+// One electron per lane.  The FP32 transport keeps the energy as an unevaluated
+// float pair (hi + lo, compensated subtraction) so every energy threshold
+// (absorption, channel thresholds, table rows, the binary-collision limit) is
+// decided on an energy that agrees with an FP64 run to ~48 bits.  The channel
+// pick reads an LDS-staged table of cumulative channel probabilities
+// interpolated in ln E and falls back to the exact FP64 pick whenever the
+// random number lies within the table's guard band of a channel boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mcd.h"
+
+namespace mcd {
+
+constexpr double kPiEFour = 6.51408491409531e-14;  // cm^2 eV^2, parameters.py:7
+constexpr double kBohrR = 5.29177210903e-11;       // m, parameters.py:8
+constexpr double kPi = 3.141592653589793;
+constexpr float kPiF = 3.14159265358979f;
+constexpr float kMasked = 2.0f;
+constexpr int64_t kMaxStepsOff = 1ll << 24;  // step cap when maxScatt is off  // device sentinel for masked CDF entries (> any u)
+
+// Pick-table node layout: [B_0 .. B_{n-2}, ln mfp, perm_lo, perm_hi|meta, pad..]
+// perm: 4-bit channel labels in sorted (pick) order, positions 0..7 in perm_lo,
+// 8..10 in bits 0..11 of perm_hi.  meta (bits 12..): cell [k, k+1] swap position
+// (bits 12..15, 15 = no swap) and exact-only flag (bit 16).
+constexpr uint32_t kMetaNoSwap = 15u;
+constexpr uint32_t kMetaExact = 1u << 16;
+
+struct DevTable {
+  const float* Es;   // nE energies (float32 grid)
+  const float* Ws;   // nW losses
+  const float* cdf;  // nE x (nW-1), masked = kMasked
+  int nE, nW;
+  float E_first, inv_h;  // linspace index guess
+};
+
+struct DevPick {
+  const float* nodes;  // n_nodes x stride floats (global; staged into LDS)
+  int n_nodes, stride, n_ch;
+  float lnE0, inv_dlnE;  // node k sits at ln E = lnE0 + k / inv_dlnE
+  float delta;           // guard band on the cumulative probabilities
+};
+
+// Per-configuration constants, prepared once on the host in FP64.
+struct DevCfg {
+  int mode, bethe, n_shells, Z;
+  uint32_t config_id;
+  int64_t max_scatt;
+  // FP64 copies (FP64 transport and the exact pick)
+  double E0, Emin, Wc, Ef, Epl, atnd, thickness, c_alpha, ZZ, n_val;
+  double J, bethe_kk, zeff, lnEstar, E_val;
+  double Eb[MCD_MAX_SHELLS], ns[MCD_MAX_SHELLS];
+  double dir0[3];
+  // FP32 constants
+  float E0_hi, E0_lo;
+  float Emin_hi, Emin_lo, Ef2_hi, Ef2_lo;  // Ef2 = E_F (pair)
+  float Epl_hi, Epl_lo;
+  float thickness_f, c_alpha_f, ruth_k_f, mfp_k_f;
+  float bethe_k1_f, bethe_c1_f, bethe_kJ_f, expl_k_f, expl_inv_Estar_f;
+  float dir0_f[3];
+  DevTable tm, tg;
+  DevPick pick;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and the (0,1) mapping.
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                       uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+  }
+  return {c0, c1, c2, c3};
+}
+
+// u = (2*(x>>9)+1) * 2^-24, exact in FP32.
+__device__ __forceinline__ float u01f(uint32_t x) { return (float)(2u * (x >> 9) + 1u) * 5.9604644775390625e-08f; }
+__device__ __forceinline__ double u01d(uint32_t x) { return (double)(2u * (x >> 9) + 1u) * 5.9604644775390625e-08; }
+
+// ---------------------------------------------------------------------------
+// Float-pair energy arithmetic.
+struct F2 { float hi, lo; };
+
+__device__ __forceinline__ F2 f2_from_double(double v) {
+  float h = (float)v;
+  return {h, (float)(v - (double)h)};
+}
+
+// a - b, TwoSum on the high parts then Fast2Sum renormalisation.
+__device__ __forceinline__ F2 f2_sub(F2 a, float bh, float bl) {
+  float s = a.hi - bh;
+  float bb = s - a.hi;
+  float err = (a.hi - (s - bb)) + (-bh - bb);
+  float lo = (a.lo - bl) + err;
+  float hi = s + lo;
+  lo = lo - (hi - s);
+  return {hi, lo};
+}
+
+// a <= b for normalised pairs
+__device__ __forceinline__ bool f2_le(F2 a, float bh, float bl) { return a.hi < bh || (a.hi == bh && a.lo <= bl); }
+// v < a for a float v
+__device__ __forceinline__ bool f_lt_f2(float v, F2 a) { return v < a.hi || (v == a.hi && a.lo > 0.f); }
+
+// ---------------------------------------------------------------------------
+// FP64 cross sections, mirroring crossSections.py operation order.
+#pragma clang fp contract(off)
+__device__ __forceinline__ double d_sigma_ruth(double E, const DevCfg& c) {  // crossSections.py:37-58
+  double a = c.c_alpha / E;
+  double Ek = E * 1e-3;
+  double r = (Ek + 511.) / (Ek + 1024.);
+  return 5.21e-21 * (c.ZZ / (Ek * Ek)) * (r * r) * (4. * kPi) / (a * (1. + a));
+}
+__device__ __forceinline__ double d_sigma_moller(double E, const DevCfg& c) {  // :183-208
+  if (E <= c.Wc) return 0.;
+  double eps = c.Wc * 1. / E;
+  return c.n_val * kPiEFour * (1. / (E * E)) * (1. / eps - 1. / (1. - eps) + log(eps / (1. - eps)));
+}
+__device__ __forceinline__ double d_sigma_gryz(double E, double Esi, double nsi) {  // :213-242
+  if (E <= Esi) return 0.;
+  double U = E * 1. / Esi;
+  return nsi * kPiEFour * pow((U - 1.) / (U + 1.), 1.5) *
+         (1. + (2. * (1. - (1. / (2. * U)))) / 3. * log(2.7 + pow(U - 1., 0.5))) / (E * Esi);
+}
+__device__ __forceinline__ double d_sigma_quinn(double E, const DevCfg& c) {  // :247-284
+  if (E <= c.Epl) return 0.;
+  double E_Ef = E * 1. / c.Ef;
+  double Epl_Ef = c.Epl * 1. / c.Ef;
+  double tot = c.Epl * log((pow(1. + Epl_Ef, 0.5) - 1.) / (pow(E_Ef, 0.5) - pow(E_Ef - Epl_Ef, 0.5))) /
+               (2. * kBohrR * E);
+  return (tot / c.atnd) * 1e4;
+}
+
+// Insertion-ordered sigma set (scattering.py:480-508); returns the count.
+__device__ __forceinline__ int d_sigma_set(double E, const DevCfg& c, double* sig, int* lab) {
+  int k = 0;
+  sig[k] = d_sigma_ruth(E, c); lab[k++] = MCD_CH_RUTH;
+  sig[k] = d_sigma_moller(E, c); lab[k++] = MCD_CH_MOLLER;
+  for (int s = 0; s < c.n_shells; ++s) { sig[k] = d_sigma_gryz(E, c.Eb[s], c.ns[s]); lab[k++] = MCD_CH_GRYZ0 + s; }
+  sig[k] = d_sigma_quinn(E, c); lab[k++] = MCD_CH_QUINN;
+  return k;
+}
+
+// Sorted (pick-order) cumulative probabilities: stable descending sort,
+// cumulative sum / sequential total (pickFromSigmas, scattering.py:39-59).
+// Writes order[] (insertion indices) and probs[] and returns the total sigma.
+__device__ __forceinline__ double d_sorted_probs(const double* sig, int n, int* order, double* probs) {
+  for (int i = 0; i < n; ++i) order[i] = i;
+  for (int i = 1; i < n; ++i) {
+    int v = order[i], j = i - 1;
+    while (j >= 0 && sig[order[j]] < sig[v]) { order[j + 1] = order[j]; --j; }
+    order[j + 1] = v;
+  }
+  double acc = 0., tot = 0.;
+  for (int i = 0; i < n; ++i) { acc += sig[order[i]]; probs[i] = acc; }
+  for (int i = 0; i < n; ++i) tot += sig[order[i]];
+  for (int i = 0; i < n; ++i) probs[i] = probs[i] / tot;
+  return tot;
+}
+
+// Exact FP64 pick: returns the channel label; also the FP64 mean free path.
+__device__ __noinline__ int d_pick(double E, double u, const DevCfg& c, double* mfp_out) {
+  double sig[MCD_MAX_CHANNELS], probs[MCD_MAX_CHANNELS];
+  int lab[MCD_MAX_CHANNELS], order[MCD_MAX_CHANNELS];
+  int n = d_sigma_set(E, c, sig, lab);
+  double s = 0.;
+  for (int i = 0; i < n; ++i) s += sig[i];
+  *mfp_out = 1. / (c.atnd * s) * 1e14;  // mfp_from_sigma, scattering.py:19-36
+  d_sorted_probs(sig, n, order, probs);
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (probs[mid] < u) lo = mid + 1; else hi = mid;
+  }
+  if (lo >= n) lo = n - 1;
+  return lab[order[lo]];
+}
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------
+// Interpolated pick (FP32).  Node k holds the FP64 sorted cumulative
+// probabilities at E_k; within cell [k, k+1] the order is constant except in
+// cells flagged with an adjacent swap (a sigma crossing), where the two
+// swapping channels' probabilities are interpolated separately and compared.
+template <int NCH>
+struct PickEval {
+  float B[NCH - 1];      // cumulative probabilities in pick order (last = 1 implied)
+  uint32_t perm_lo, perm_hi;
+  float pA, pB;          // swap cells: interpolated probabilities of the two channels
+  int swap;              // swap position or kMetaNoSwap
+  bool exact;            // exact-only cell
+  float mfp;
+};
+
+template <int NCH>
+__device__ __forceinline__ void f_pick_eval(const float* __restrict__ nodes, const DevPick& P, float E,
+                                            PickEval<NCH>& o) {
+  constexpr int NB = NCH - 1;
+  float x = (__logf(E) - P.lnE0) * P.inv_dlnE;
+  x = fminf(fmaxf(x, 0.f), (float)(P.n_nodes - 1));
+  const int k = min((int)x, P.n_nodes - 2);
+  const float f = x - (float)k;
+  const float* a = nodes + k * P.stride;
+  const float* b = a + P.stride;
+  o.mfp = __expf(a[NB] + f * (b[NB] - a[NB]));
+  o.perm_lo = __float_as_uint(a[NB + 1]);
+  const uint32_t meta = __float_as_uint(a[NB + 2]);
+  o.perm_hi = meta & 0xFFFu;
+  o.exact = (meta & kMetaExact) != 0;
+  o.swap = (int)((meta >> 12) & 15u);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) o.B[i] = a[i] + f * (b[i] - a[i]);
+  o.pA = o.pB = 0.f;
+  if (o.swap != (int)kMetaNoSwap) {
+    // node k: channel A at position j, B at j+1; node k+1: B at j, A at j+1
+    const int j = o.swap;
+    const float am = j > 0 ? a[j - 1] : 0.f, bm = j > 0 ? b[j - 1] : 0.f;
+    const float ap = j + 1 < NB ? a[j + 1] : 1.f, bp = j + 1 < NB ? b[j + 1] : 1.f;
+    const float pA_k = a[j] - am, pB_k = ap - a[j];
+    const float pB_k1 = b[j] - bm, pA_k1 = bp - b[j];
+    o.pA = pA_k + f * (pA_k1 - pA_k);
+    o.pB = pB_k + f * (pB_k1 - pB_k);
+    const float prev = j > 0 ? o.B[j - 1] : 0.f;
+    if (o.pB > o.pA) {
+      o.B[j] = prev + o.pB;
+      uint64_t perm = ((uint64_t)o.perm_hi << 32) | o.perm_lo;
+      const uint64_t la = (perm >> (4 * j)) & 15u, lb = (perm >> (4 * (j + 1))) & 15u;
+      perm &= ~((uint64_t)0xFFu << (4 * j));
+      perm |= (lb << (4 * j)) | (la << (4 * (j + 1)));
+      o.perm_lo = (uint32_t)perm;
+      o.perm_hi = (uint32_t)(perm >> 32);
+    } else {
+      o.B[j] = prev + o.pA;
+    }
+  }
+}
+
+__device__ __forceinline__ int perm_label(uint32_t lo, uint32_t hi, int pos) {
+  return (int)((pos < 8 ? (lo >> (4 * pos)) : (hi >> (4 * (pos - 8)))) & 15u);
+}
+
+// Returns the channel label, or -1 when the guard band (or an exact-only cell)
+// requires the FP64 pick.  mfp is always set.
+template <int NCH>
+__device__ __forceinline__ int f_pick_table(const float* __restrict__ nodes, const DevPick& P, float E, float u,
+                                            float& mfp) {
+  PickEval<NCH> o;
+  f_pick_eval<NCH>(nodes, P, E, o);
+  mfp = o.mfp;
+  if (o.exact) return -1;
+  if (o.swap != (int)kMetaNoSwap && fabsf(o.pA - o.pB) <= P.delta) return -1;
+  int cnt = 0;
+  float gap = 1.f;
+#pragma unroll
+  for (int i = 0; i < NCH - 1; ++i) {
+    cnt += (o.B[i] < u) ? 1 : 0;
+    gap = fminf(gap, fabsf(o.B[i] - u));
+  }
+  if (gap <= P.delta) return -1;
+  return perm_label(o.perm_lo, o.perm_hi, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// Inverse-CDF table lookup (pickTable, scattering.py:63-94).
+// Row = first grid energy >= E (bisect_left), column = first CDF entry >= u.
+template <class EnergyLess>
+__device__ __forceinline__ int table_row(const DevTable& t, float Eguess, EnergyLess es_lt_E) {
+  int i = (int)ceilf((Eguess - t.E_first) * t.inv_h);
+  i = max(0, min(i, t.nE - 1));
+  // settle bisect_left exactly: Es[i-1] < E <= Es[i]
+  int it = 0;
+  while (it < 4 && i > 0 && !es_lt_E(t.Es[i - 1])) { --i; ++it; }
+  while (it < 8 && i < t.nE && es_lt_E(t.Es[i])) { ++i; ++it; }
+  bool ok = (i == 0 || es_lt_E(t.Es[i - 1])) && (i == t.nE || !es_lt_E(t.Es[i]));
+  if (!ok) {
+    int lo = 0, hi = t.nE;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (es_lt_E(t.Es[mid])) lo = mid + 1; else hi = mid;
+    }
+    i = lo;
+  }
+  return min(i, t.nE - 1);
+}
+
+__device__ __forceinline__ int table_col(const DevTable& t, int row, float u) {
+  const float* r = t.cdf + (size_t)row * (size_t)(t.nW - 1);
+  int lo = 0, hi = t.nW - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (r[mid] < u) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Direction update, closed form of q_az * q_polar acting on (d, y_local)
+// (electron.py:87-99, rotation.py:41-53): with w = y x d,
+//   d' = cos(th) d + sin(th) (cos(ph) w + sin(ph) y),  y' = cos(ph) y - sin(ph) w,
+// cos(th) = c2 - s2, sin(th) = 2 sqrt(c2 s2); the azimuth fold
+// c_hPhi = sqrt(1 - sin^2(pi u)) gives cos(ph) = cos(2 pi u), sin(ph) = |sin(2 pi u)|.
+__device__ __forceinline__ void f_rotate(float c2, float s2, float u_az, float* d, float* y) {
+  float ct = c2 - s2;
+  float st = 2.f * __builtin_amdgcn_sqrtf(c2 * s2);
+  float sp, cp;
+  sincospif(2.f * u_az, &sp, &cp);
+  sp = fabsf(sp);
+  float w0 = y[1] * d[2] - y[2] * d[1];
+  float w1 = y[2] * d[0] - y[0] * d[2];
+  float w2 = y[0] * d[1] - y[1] * d[0];
+  float nd0 = ct * d[0] + st * (cp * w0 + sp * y[0]);
+  float nd1 = ct * d[1] + st * (cp * w1 + sp * y[1]);
+  float nd2 = ct * d[2] + st * (cp * w2 + sp * y[2]);
+  float ny0 = cp * y[0] - sp * w0;
+  float ny1 = cp * y[1] - sp * w1;
+  float ny2 = cp * y[2] - sp * w2;
+  float rd = __builtin_amdgcn_rsqf(nd0 * nd0 + nd1 * nd1 + nd2 * nd2);
+  float ry = __builtin_amdgcn_rsqf(ny0 * ny0 + ny1 * ny1 + ny2 * ny2);
+  d[0] = nd0 * rd; d[1] = nd1 * rd; d[2] = nd2 * rd;
+  y[0] = ny0 * ry; y[1] = ny1 * ry; y[2] = ny2 * ry;
+}
+
+#pragma clang fp contract(off)
+// FP64 rotation as the reference composes it: Hamilton product, rotation
+// matrix normalised by |q|^2, renormalisation of both vectors.
+__device__ __forceinline__ void d_rotate(double c2, double halfPhi, double* d, double* y) {
+  double s_ht = pow(1. - c2, 0.5), c_ht = pow(c2, 0.5);
+  double s_hp = sin(halfPhi), c_hp = pow(1. - s_hp * s_hp, 0.5);
+  double a1 = c_hp, b1 = d[0] * s_hp, c1 = d[1] * s_hp, d1 = d[2] * s_hp;
+  double a2 = c_ht, b2 = y[0] * s_ht, c2q = y[1] * s_ht, d2 = y[2] * s_ht;
+  double w = a1 * a2 - b1 * b2 - c1 * c2q - d1 * d2;
+  double x = a1 * b2 + b1 * a2 + c1 * d2 - d1 * c2q;
+  double yy = a1 * c2q - b1 * d2 + c1 * a2 + d1 * b2;
+  double z = a1 * d2 + b1 * c2q - c1 * b2 + d1 * a2;
+  double n = w * w + x * x + yy * yy + z * z;
+  double M00 = 1.0 - 2 * (yy * yy + z * z) / n, M01 = 2 * (x * yy - z * w) / n, M02 = 2 * (x * z + yy * w) / n;
+  double M10 = 2 * (x * yy + z * w) / n, M11 = 1.0 - 2 * (x * x + z * z) / n, M12 = 2 * (yy * z - x * w) / n;
+  double M20 = 2 * (x * z - yy * w) / n, M21 = 2 * (yy * z + x * w) / n, M22 = 1.0 - 2 * (x * x + yy * yy) / n;
+  double nd0 = M00 * d[0] + M01 * d[1] + M02 * d[2];
+  double nd1 = M10 * d[0] + M11 * d[1] + M12 * d[2];
+  double nd2 = M20 * d[0] + M21 * d[1] + M22 * d[2];
+  double ny0 = M00 * y[0] + M01 * y[1] + M02 * y[2];
+  double ny1 = M10 * y[0] + M11 * y[1] + M12 * y[2];
+  double ny2 = M20 * y[0] + M21 * y[1] + M22 * y[2];
+  double ld = sqrt(nd0 * nd0 + nd1 * nd1 + nd2 * nd2);
+  double ly = sqrt(ny0 * ny0 + ny1 * ny1 + ny2 * ny2);
+  d[0] = nd0 / ld; d[1] = nd1 / ld; d[2] = nd2 / ld;
+  y[0] = ny0 / ly; y[1] = ny1 / ly; y[2] = ny2 / ly;
+}
+#pragma clang fp contract(on)
+
+}  // namespace mcd

@@ -1,0 +1,34 @@
+// mcd_internal.h — declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mcd_device.h"
+
+namespace mcd {
+
+struct RunArgs {
+  uint64_t seed;
+  uint64_t first, n;
+  mcd_electron_rec* els;
+  mcd_scatter_rec* rows;
+  uint64_t row_cap, row_max_electron;
+  unsigned long long* row_cursor;
+  unsigned long long* counters;
+  unsigned long long* queue;
+};
+
+hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int grid, int block, hipStream_t s);
+int max_blocks_per_cu(const DevCfg& c, int precision, int block);
+
+// tables
+hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, double Ef, int kind, double n_el,
+                            double Eb, int precision, float* cdf, hipStream_t s);
+hipError_t launch_pick_nodes(const DevCfg& c, int n_nodes, double lnE0, double dln, float* nodes, int stride,
+                             hipStream_t s);
+hipError_t launch_pick_check(const DevCfg& c, int n_checks, float* err, int* bad, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, int n, hipStream_t s);
+hipError_t launch_unmask(const float* src, float* dst, size_t n, hipStream_t s);
+hipError_t launch_mask(const float* src, float* dst, size_t n, hipStream_t s);
+
+}  // namespace mcd

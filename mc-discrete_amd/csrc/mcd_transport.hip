@@ -1,0 +1,451 @@
+// mcd_transport.hip — persistent electron-transport kernels for gfx950.
+//
+// One electron per lane.  Each wavefront refills its idle lanes from a device
+// work counter (ballot + popcount prefix, one atomic per refill), so lanes
+// whose electron was absorbed or backscattered pick up the next electron
+// immediately instead of idling until the wavefront's longest trajectory ends.
+// The step follows trajectory_DS (MC/tools/singleScatter.py:10-41) and
+// trajectory_cont_cl (singleScatter.py:205-239); see mcd_device.h for the
+// numerics.
+#include <hip/hip_runtime.h>
+
+#include "mcd_device.h"
+#include "mcd_internal.h"
+
+namespace mcd {
+
+__device__ __forceinline__ void write_rec(mcd_electron_rec* rec, float E, const float* d, const float* p, uint32_t nsc,
+                                          int o1, int o2, uint32_t flags) {
+  rec->final_E = E;
+  rec->dir[0] = d[0]; rec->dir[1] = d[1]; rec->dir[2] = d[2];
+  rec->pos[0] = p[0]; rec->pos[1] = p[1]; rec->pos[2] = p[2];
+  rec->n_scatt = nsc;
+  rec->outcome = (uint8_t)o1;
+  rec->outcome2 = (uint8_t)o2;
+  rec->flags = (uint8_t)flags;
+  rec->pad = 0;
+}
+
+// wave-level append of scatter rows; returns the slot or ~0 when full
+__device__ __forceinline__ uint64_t append_slot(bool want, unsigned long long* cursor, uint64_t cap, int lane) {
+  uint64_t m = __ballot(want);
+  if (!m) return ~0ull;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(cursor, (unsigned long long)__popcll(m));
+  uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+  base = ((uint64_t)hi << 32) | lo;
+  uint64_t slot = base + __popcll(m & ((1ull << lane) - 1ull));
+  return (want && slot < cap) ? slot : ~0ull;
+}
+
+__device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const uint64_t* v, int nv, int lane) {
+  for (int i = 0; i < nv; ++i) {
+    uint64_t x = v[i];
+    for (int off = 32; off > 0; off >>= 1) {
+      uint32_t lo = __shfl_xor((uint32_t)x, off), hi = __shfl_xor((uint32_t)(x >> 32), off);
+      x += ((uint64_t)hi << 32) | lo;
+    }
+    if (lane == 0 && x) atomicAdd(&cnt[i], (unsigned long long)x);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 transport (DS and continuous).
+template <int MODE, int NCH>
+__global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
+  extern __shared__ float lds[];
+  if (MODE == MCD_MODE_DS) {
+    const int nf = c.pick.n_nodes * c.pick.stride;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
+  const uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
+
+  bool active = false, open = true;
+  uint64_t e = 0;
+  uint32_t step = 0, flags = 0;
+  F2 E = {0.f, 0.f};
+  float p[3] = {0.f, 0.f, 0.f}, pc[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
+  // counters: events, electrons, bks, abs, trsm, long, fp64 picks
+  uint64_t cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+
+  while (true) {
+    if (open) {
+      uint64_t idle = __ballot(!active);
+      if (idle) {
+        int nidle = __popcll(idle);
+        int leader = __ffsll((unsigned long long)idle) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(R.queue, (unsigned long long)nidle);
+        uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+        base = ((uint64_t)hi << 32) | lo;
+        if (!active) {
+          uint64_t idx = base + __popcll(idle & ((1ull << lane) - 1ull));
+          if (idx < R.n) {
+            active = true;
+            e = R.first + idx;
+            step = 0;
+            flags = 0;
+            E.hi = c.E0_hi; E.lo = c.E0_lo;
+            p[0] = p[1] = p[2] = 0.f;
+            pc[0] = pc[1] = pc[2] = 0.f;
+            d[0] = c.dir0_f[0]; d[1] = c.dir0_f[1]; d[2] = c.dir0_f[2];
+            y[0] = 0.f; y[1] = 1.f; y[2] = 0.f;
+          }
+        }
+        if (base + (uint64_t)nidle >= R.n) open = false;
+      }
+    }
+    if (!__any(active)) break;
+    if (!active) continue;
+
+    // ---- one loop iteration of trajectory_DS / trajectory_cont_cl ----
+    if ((int64_t)step >= c.max_scatt) {  // num_scatt > maxScatt -> 'long'
+      write_rec(&R.els[e - R.first], E.hi, d, p, step, MCD_OUT_LONG, 0, flags);
+      cnt[1]++; cnt[5]++;
+      active = false;
+      continue;
+    }
+    const U4 r = philox10((uint32_t)e, c.config_id | (uint32_t)(e >> 32) << 16, step, 0u, k0, k1);
+    const F2 Epre = E;
+    float l, Wh, Wl = 0.f, c2, s2, u_az;
+    int ch = MCD_CH_RUTH;
+    uint32_t sflags = 0;
+    if (MODE == MCD_MODE_DS) {
+      const float u0 = u01f(r.x), u1 = u01f(r.y), u2 = u01f(r.z);
+      u_az = u01f(r.w);
+      float mfp;
+      ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp);
+      if (ch < 0) {  // guard band: settle the pick in FP64
+        double m64;
+        ch = d_pick((double)E.hi + (double)E.lo, (double)u0, c, &m64);
+        mfp = (float)m64;
+        sflags |= MCD_FLAG_FP64_PICK;
+        cnt[6]++;
+      }
+      l = -mfp * __logf(u1);
+      if (ch == MCD_CH_RUTH) {  // Rutherford_halfPol, scattering.py:125-152
+        Wh = 0.f;
+        const float a = c.c_alpha_f * __builtin_amdgcn_rcpf(E.hi);
+        const float om = 1.f - u2;
+        const float inv = __builtin_amdgcn_rcpf(om + a);
+        s2 = a * u2 * inv;
+        c2 = om * (1.f + a) * inv;
+      } else if (ch == MCD_CH_QUINN) {  // scattering.py:643-656
+        Wh = c.Epl_hi; Wl = c.Epl_lo;
+        c2 = 1.f; s2 = 0.f;
+      } else {  // Moller table, or the outer-shell Gryzinski table for all shells (:602-640)
+        const DevTable& t = (ch == MCD_CH_MOLLER) ? c.tm : c.tg;
+        const F2 Ec = E;
+        const int row = table_row(t, E.hi, [Ec](float v) { return f_lt_f2(v, Ec); });
+        const int col = table_col(t, row, u2);
+        Wh = t.Ws[col];
+        if (!f_lt_f2(Wh, E)) sflags |= MCD_FLAG_LOSS_GE_E;
+        // binaryCollModel (:156-179): W <= (E - E_F)/2
+        const F2 D = f2_sub(E, c.Ef2_hi, c.Ef2_lo);
+        const float w2 = 2.f * Wh;
+        const bool ok = !(D.hi < w2 || (D.hi == w2 && D.lo < 0.f));
+        if (ok) {
+          const float x = Wh * __builtin_amdgcn_rcpf(E.hi);
+          const float sq = __builtin_amdgcn_sqrtf(1.f - x);
+          c2 = 0.5f * (1.f + sq);
+          s2 = 0.5f * x * __builtin_amdgcn_rcpf(1.f + sq);
+        } else {
+          c2 = s2 = __builtin_nanf("");
+          sflags |= MCD_FLAG_NAN_DIR;
+        }
+      }
+    } else {
+      const float u0 = u01f(r.x), u1 = u01f(r.y);
+      u_az = u01f(r.z);
+      // Rutherford-only mean free path (scattering.py:780-797)
+      const float a = c.c_alpha_f * __builtin_amdgcn_rcpf(E.hi);
+      const float Ek = E.hi * 1e-3f;
+      const float rr = (Ek + 511.f) * __builtin_amdgcn_rcpf(Ek + 1024.f);
+      const float sig = c.ruth_k_f * rr * rr * __builtin_amdgcn_rcpf(Ek * Ek * a * (1.f + a));
+      const float mfp = c.mfp_k_f * __builtin_amdgcn_rcpf(sig);
+      l = -mfp * __logf(u0);
+      if (!(l < 1e4f)) sflags |= MCD_FLAG_LONG_PATH;
+      // Bethe stopping power at the pre-step energy (stoppingPowers.py:112-215)
+      float S;
+      const float iE = __builtin_amdgcn_rcpf(E.hi);
+      if (c.bethe == MCD_BETHE_CLASSICAL) S = c.bethe_k1_f * iE * __logf(c.bethe_c1_f * E.hi);
+      else if (c.bethe == MCD_BETHE_JL) S = c.bethe_k1_f * iE * __logf(c.bethe_c1_f * (E.hi + c.bethe_kJ_f));
+      else S = c.expl_k_f * iE * __logf(E.hi * c.expl_inv_Estar_f);
+      Wh = l * S;
+      if (!f_lt_f2(Wh, E)) sflags |= MCD_FLAG_LOSS_GE_E;
+      const float om = 1.f - u1;
+      const float inv = __builtin_amdgcn_rcpf(om + a);
+      s2 = a * u1 * inv;
+      c2 = om * (1.f + a) * inv;
+    }
+    cnt[0]++;
+
+    // update_xyz (electron.py:64-84), Kahan-compensated position
+    const float v0 = l * d[0] - pc[0], v1 = l * d[1] - pc[1], v2 = l * d[2] - pc[2];
+    const float t0 = p[0] + v0, t1 = p[1] + v1, t2 = p[2] + v2;
+    int o1 = MCD_OUT_NONE, o2 = MCD_OUT_NONE;
+    const float pre0 = p[0], pre1 = p[1], pre2 = p[2];
+    if (t2 <= 0.f) o1 = MCD_OUT_BKS;
+    else if (t2 >= c.thickness_f) o1 = MCD_OUT_TRSM;
+    else {
+      pc[0] = (t0 - p[0]) - v0; pc[1] = (t1 - p[1]) - v1; pc[2] = (t2 - p[2]) - v2;
+      p[0] = t0; p[1] = t1; p[2] = t2;
+    }
+    // update_energy (electron.py:44-59): absorbed when E - W <= Emin
+    E = f2_sub(E, Wh, Wl);
+    if (f2_le(E, c.Emin_hi, c.Emin_lo)) {
+      if (o1 == MCD_OUT_NONE) o1 = MCD_OUT_ABS; else o2 = MCD_OUT_ABS;
+    }
+    flags |= sflags;
+    const bool done = o1 != MCD_OUT_NONE;
+    if (done) {
+      // saveOutcomes: pre-loss energy, pre-rotation direction, last interior position
+      const float sp[3] = {p[0] - pc[0], p[1] - pc[1], p[2] - pc[2]};
+      write_rec(&R.els[e - R.first], Epre.hi, d, sp, step + 1, o1, o2, flags);
+      cnt[1]++;
+      cnt[2] += (o1 == MCD_OUT_BKS);
+      cnt[3] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
+      cnt[4] += (o1 == MCD_OUT_TRSM);
+    }
+    const bool rec_row = R.rows != nullptr && e < R.row_max_electron;
+    if (!done || rec_row) f_rotate(c2, s2, u_az, d, y);  // update_direction
+    if (R.rows != nullptr) {
+      const uint64_t slot = append_slot(rec_row, R.row_cursor, R.row_cap, lane);
+      if (slot != ~0ull) {
+        mcd_scatter_rec* w = &R.rows[slot];
+        w->electron = (uint32_t)e; w->step = step; w->config = (uint16_t)c.config_id;
+        w->channel = (uint8_t)ch; w->outcome = (uint8_t)(o2 ? o2 : o1);
+        w->E = Epre.hi; w->E_loss = Wh + Wl; w->pol = c2; w->az = kPiF * u_az; w->pathl = l;
+        w->x = pre0; w->y = pre1; w->z = pre2;
+        w->energy = E.hi;
+        w->dx = d[0]; w->dy = d[1]; w->dz = d[2];
+        w->step_flags = sflags;
+      }
+    }
+    step++;
+    if (done) active = false;
+  }
+  wave_add_counters(R.counters, cnt, 7, lane);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 transport: the same loop with every quantity in double and the exact
+// pick each step (the all-FP64 GPU run).  Arithmetic is not contracted into
+// FMAs so the operation order matches the reference expressions.
+struct DStep {
+  double l, W, c2, hp;
+  int ch;
+  uint32_t sflags;
+};
+
+#pragma clang fp contract(off)
+template <int MODE>
+__device__ __forceinline__ DStep d_step_physics(const DevCfg& c, double E, const U4 r) {
+  DStep o;
+  o.sflags = 0;
+  o.ch = MCD_CH_RUTH;
+  if (MODE == MCD_MODE_DS) {
+    const double u0 = u01d(r.x), u1 = u01d(r.y), u2 = u01d(r.z), u3 = u01d(r.w);
+    double mfp;
+    o.ch = d_pick(E, u0, c, &mfp);          // det_type, scattering.py:530-549
+    o.l = -mfp * log(u1);                   // compute_pathl, :551-565
+    if (o.ch == MCD_CH_RUTH) {              // :575-586
+      o.W = 0.;
+      const double a = c.c_alpha / E;
+      o.c2 = 1. - (a * u2 / (1. + a - u2));
+    } else if (o.ch == MCD_CH_QUINN) {      // :643-656
+      o.W = c.Epl;
+      o.c2 = 1.;
+    } else {                                // :602-640
+      const DevTable& t = (o.ch == MCD_CH_MOLLER) ? c.tm : c.tg;
+      const double Ec = E;
+      const int row = table_row(t, (float)E, [Ec](float v) { return (double)v < Ec; });
+      const int col = table_col(t, row, (float)u2);
+      o.W = (double)t.Ws[col];
+      if (!(o.W < E)) o.sflags |= MCD_FLAG_LOSS_GE_E;
+      if (o.W <= (E - c.Ef) * 0.5) o.c2 = 0.5 * (sqrt(1 - (o.W / E)) + 1);
+      else { o.c2 = __builtin_nan(""); o.sflags |= MCD_FLAG_NAN_DIR; }
+    }
+    o.hp = kPi * u3;
+  } else {
+    const double u0 = u01d(r.x), u1 = u01d(r.y), u2 = u01d(r.z);
+    const double sr = d_sigma_ruth(E, c);
+    const double mfp = 1. / (c.atnd * sr) * 1e14;  // scattering.py:792
+    o.l = -mfp * log(u0);                          // :802-813
+    if (!(o.l < 1e4)) o.sflags |= MCD_FLAG_LONG_PATH;
+    double S;                                      // stoppingPowers.py:112-215
+    if (c.bethe == MCD_BETHE_CLASSICAL) S = 2. * kPiEFour * c.atnd * ((double)c.Z / E) * log(1.166 * (E) / c.J) * 1e-14;
+    else if (c.bethe == MCD_BETHE_JL)
+      S = 2. * kPiEFour * c.atnd * ((double)c.Z / E) * log(1.166 * (E + c.bethe_kk * c.J) / c.J) * 1e-14;
+    else {
+      const double pref = (2. * kPiEFour * c.atnd / E) * 1e-14;
+      double sumi = c.n_val * log(E * 1. / c.E_val);
+      for (int i = 0; i < c.n_shells; ++i) sumi = sumi + c.ns[i] * log(E * 1. / c.Eb[i]);
+      S = pref * sumi;
+    }
+    o.W = o.l * S;                                 // compute_Eloss, :815-826
+    if (!(o.W < E)) o.sflags |= MCD_FLAG_LOSS_GE_E;
+    const double a = c.c_alpha / E;                // compute_sAngles, :828-840
+    o.c2 = 1. - (a * u1 / (1. + a - u1));
+    o.hp = kPi * u2;
+  }
+  return o;
+}
+
+__device__ __forceinline__ void d_advance(const double* p, double l, const double* d, double* np_) {
+  for (int i = 0; i < 3; ++i) np_[i] = p[i] + l * d[i];
+}
+#pragma clang fp contract(on)
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
+  bool active = false, open = true;
+  uint64_t e = 0;
+  uint32_t step = 0, flags = 0;
+  double E = 0., p[3] = {0., 0., 0.}, d[3] = {0., 0., 0.}, y[3] = {0., 0., 0.};
+  uint64_t cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  while (true) {
+    if (open) {
+      uint64_t idle = __ballot(!active);
+      if (idle) {
+        int nidle = __popcll(idle);
+        int leader = __ffsll((unsigned long long)idle) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(R.queue, (unsigned long long)nidle);
+        uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+        base = ((uint64_t)hi << 32) | lo;
+        if (!active) {
+          uint64_t idx = base + __popcll(idle & ((1ull << lane) - 1ull));
+          if (idx < R.n) {
+            active = true;
+            e = R.first + idx;
+            step = 0; flags = 0;
+            E = c.E0;
+            p[0] = p[1] = p[2] = 0.;
+            d[0] = c.dir0[0]; d[1] = c.dir0[1]; d[2] = c.dir0[2];
+            y[0] = 0.; y[1] = 1.; y[2] = 0.;
+          }
+        }
+        if (base + (uint64_t)nidle >= R.n) open = false;
+      }
+    }
+    if (!__any(active)) break;
+    if (!active) continue;
+    if ((int64_t)step >= c.max_scatt) {
+      const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+      const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
+      write_rec(&R.els[e - R.first], (float)E, df, pf, step, MCD_OUT_LONG, 0, flags);
+      cnt[1]++; cnt[5]++;
+      active = false;
+      continue;
+    }
+    const U4 r = philox10((uint32_t)e, c.config_id | (uint32_t)(e >> 32) << 16, step, 0u, k0, k1);
+    const double Epre = E;
+    const DStep o = d_step_physics<MODE>(c, E, r);
+    cnt[0]++;
+    double np_[3];
+    d_advance(p, o.l, d, np_);
+    int o1 = MCD_OUT_NONE, o2 = MCD_OUT_NONE;
+    const double pre0 = p[0], pre1 = p[1], pre2 = p[2];
+    if (np_[2] <= 0.) o1 = MCD_OUT_BKS;
+    else if (np_[2] >= c.thickness) o1 = MCD_OUT_TRSM;
+    else { p[0] = np_[0]; p[1] = np_[1]; p[2] = np_[2]; }
+    const double En = E - o.W;
+    if (En <= c.Emin) { if (o1 == MCD_OUT_NONE) o1 = MCD_OUT_ABS; else o2 = MCD_OUT_ABS; }
+    E = En;
+    uint32_t sflags = o.sflags;
+    if (o.c2 != o.c2) sflags |= MCD_FLAG_NAN_DIR;
+    flags |= sflags;
+    const bool done = o1 != MCD_OUT_NONE;
+    if (done) {
+      const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+      const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
+      write_rec(&R.els[e - R.first], (float)Epre, df, pf, step + 1, o1, o2, flags);
+      cnt[1]++;
+      cnt[2] += (o1 == MCD_OUT_BKS);
+      cnt[3] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
+      cnt[4] += (o1 == MCD_OUT_TRSM);
+    }
+    const bool rec_row = R.rows != nullptr && e < R.row_max_electron;
+    if (!done || rec_row) d_rotate(o.c2, o.hp, d, y);
+    if (R.rows != nullptr) {
+      const uint64_t slot = append_slot(rec_row, R.row_cursor, R.row_cap, lane);
+      if (slot != ~0ull) {
+        mcd_scatter_rec* w = &R.rows[slot];
+        w->electron = (uint32_t)e; w->step = step; w->config = (uint16_t)c.config_id;
+        w->channel = (uint8_t)o.ch; w->outcome = (uint8_t)(o2 ? o2 : o1);
+        w->E = (float)Epre; w->E_loss = (float)o.W; w->pol = (float)o.c2; w->az = (float)o.hp;
+        w->pathl = (float)o.l;
+        w->x = (float)pre0; w->y = (float)pre1; w->z = (float)pre2;
+        w->energy = (float)E;
+        w->dx = (float)d[0]; w->dy = (float)d[1]; w->dz = (float)d[2];
+        w->step_flags = sflags;
+      }
+    }
+    step++;
+    if (done) active = false;
+  }
+  wave_add_counters(R.counters, cnt, 7, lane);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+
+template <int MODE, int NCH>
+static hipError_t launch_f32(const DevCfg& c, const RunArgs& R, int grid, int block, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((k_transport_f32<MODE, NCH>), dim3(grid), dim3(block), lds, s, c, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int grid, int block, hipStream_t s) {
+  if (precision == MCD_PREC_FP64) {
+    if (c.mode == MCD_MODE_DS)
+      hipLaunchKernelGGL((k_transport_f64<MCD_MODE_DS>), dim3(grid), dim3(block), 0, s, c, R);
+    else
+      hipLaunchKernelGGL((k_transport_f64<MCD_MODE_CONT>), dim3(grid), dim3(block), 0, s, c, R);
+    return hipGetLastError();
+  }
+  if (c.mode == MCD_MODE_CONT) return launch_f32<MCD_MODE_CONT, 4>(c, R, grid, block, 0, s);
+  const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
+  switch (c.n_shells + 3) {
+    case 4: return launch_f32<MCD_MODE_DS, 4>(c, R, grid, block, lds, s);
+    case 5: return launch_f32<MCD_MODE_DS, 5>(c, R, grid, block, lds, s);
+    case 6: return launch_f32<MCD_MODE_DS, 6>(c, R, grid, block, lds, s);
+    case 7: return launch_f32<MCD_MODE_DS, 7>(c, R, grid, block, lds, s);
+    case 8: return launch_f32<MCD_MODE_DS, 8>(c, R, grid, block, lds, s);
+    case 9: return launch_f32<MCD_MODE_DS, 9>(c, R, grid, block, lds, s);
+    case 10: return launch_f32<MCD_MODE_DS, 10>(c, R, grid, block, lds, s);
+    case 11: return launch_f32<MCD_MODE_DS, 11>(c, R, grid, block, lds, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// occupancy query for the persistent grid
+int max_blocks_per_cu(const DevCfg& c, int precision, int block) {
+  int nb = 0;
+  hipError_t e;
+  if (precision == MCD_PREC_FP64) {
+    e = c.mode == MCD_MODE_DS
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f64<MCD_MODE_DS>, block, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f64<MCD_MODE_CONT>, block, 0);
+  } else if (c.mode == MCD_MODE_CONT) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_CONT, 4>, block, 0);
+  } else {
+    const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
+    switch (c.n_shells + 3) {
+      case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 6>, block, lds); break;
+      case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 7>, block, lds); break;
+      case 11: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 11>, block, lds); break;
+      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 7>, block, lds); break;
+    }
+  }
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+}  // namespace mcd

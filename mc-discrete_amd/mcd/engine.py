@@ -1,0 +1,152 @@
+"""Device-side transport engine: contexts, tables and runs on one GPU.
+
+A ``Transport`` holds one configuration (material, E0, Emin, Wc, tilt, mode,
+Bethe model) with its device tables and runs electron ranges through
+``mcd_forward``.  Output buffers are torch tensors on the GPU; results come back
+as numpy structured arrays (``_abi.ELECTRON_DTYPE`` / ``_abi.SCATTER_DTYPE``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._lib import check, lib
+from .material import Material
+from .tables import TableGrid, gryz_grid, moller_grid
+
+_CTX = {}
+
+
+def context(device: int = 0):
+    """Process-wide libmcd context for a device (created on first use)."""
+    if device not in _CTX:
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the transport runs only on the GPU (there is no CPU fallback)")
+        h = C.c_void_p()
+        check(lib().mcd_ctx_create(device, C.byref(h)))
+        _CTX[device] = h
+    return _CTX[device]
+
+
+@dataclass
+class RunResult:
+    els: np.ndarray              # ELECTRON_DTYPE, one record per electron
+    counters: np.ndarray         # uint64[NCOUNTERS]
+    scats: np.ndarray | None     # SCATTER_DTYPE rows sorted by (electron, step), or None
+    seconds: float | None = None
+
+    @property
+    def events(self) -> int:
+        return int(self.counters[_abi.CNT_EVENTS])
+
+
+class Transport:
+    """One configuration on one GPU."""
+
+    def __init__(self, material: str | Material, E0: float, Emin: float, Wc: float = 50.0, s_tilt: float = 0.0,
+                 mode: str = "DS", bethe_model: str = "classical", tol_E: float = 1e-3, tol_W: float = 1e-3,
+                 thickness: float = float("inf"), max_scatt: int = 0, J: float | None = None,
+                 table_precision: int = _abi.PREC_FP64, device: int = 0, grids=None, cdfs=None):
+        self.mat = material if isinstance(material, Material) else Material(material)
+        self.device = device
+        self.cfg = _abi.make_config(self.mat, E0, Emin, Wc, s_tilt, thickness, mode, bethe_model, max_scatt, J)
+        self.mode = self.cfg.mode
+        self.tol_E, self.tol_W = tol_E, tol_W
+        self.grids = grids
+        self._tables = C.c_void_p()
+        ctx = context(device)
+        if self.mode == _abi.MODE_DS:
+            if self.grids is None:
+                self.grids = (moller_grid(self.mat, E0, Emin, Wc).build_grids(tol_E, tol_W),
+                              gryz_grid(self.mat, E0, Emin).build_grids(tol_E, tol_W))
+            gm = _abi.grid_struct(self.grids[0], _abi.TABLE_MOLLER)
+            gg = _abi.grid_struct(self.grids[1], _abi.TABLE_GRYZ)
+            if cdfs is None:
+                check(lib().mcd_tables_create(ctx, C.byref(self.cfg), C.byref(gm), C.byref(gg), int(table_precision),
+                                              C.byref(self._tables)), ctx)
+            else:
+                cm = np.ascontiguousarray(cdfs[0], dtype=np.float32)
+                cg = np.ascontiguousarray(cdfs[1], dtype=np.float32)
+                check(lib().mcd_tables_upload(ctx, C.byref(self.cfg), C.byref(gm), cm.ctypes.data, C.byref(gg),
+                                              cg.ctypes.data, C.byref(self._tables)), ctx)
+        else:
+            check(lib().mcd_tables_create(ctx, C.byref(self.cfg), None, None, int(table_precision),
+                                          C.byref(self._tables)), ctx)
+
+    def __del__(self):
+        try:
+            if self._tables:
+                lib().mcd_tables_destroy(self._tables)
+                self._tables = C.c_void_p()
+        except Exception:
+            pass
+
+    # -- tables ---------------------------------------------------------------
+    def cdf(self, which: int) -> np.ndarray:
+        """Device-built CDF in the reference layout (masked = 0)."""
+        g = self.grids[which]
+        out = np.zeros((g.Es.size, g.Ws.size - 1), dtype=np.float32)
+        ctx = context(self.device)
+        check(lib().mcd_tables_download(ctx, self._tables, int(which), out.ctypes.data, out.size), ctx)
+        return out
+
+    def pick_info(self):
+        n, s, e, d = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        check(lib().mcd_tables_info(self._tables, C.byref(n), C.byref(s), C.byref(e), C.byref(d)))
+        return dict(n_nodes=n.value, swap_cells=s.value, exact_cells=e.value, delta=d.value)
+
+    # -- runs -----------------------------------------------------------------
+    def launch(self, n, els, counters, seed=1, first=0, config_id=0, scat_rows=None, scat_max_electron=0,
+               precision=_abi.PREC_FP32, block=0, waves_per_simd=0, stream=None):
+        """Asynchronous launch on `stream` into caller-owned torch buffers."""
+        ctx = context(self.device)
+        sb = None
+        if scat_rows is not None:
+            sb = _abi.ScatterBuf(scat_rows.data_ptr(), scat_rows.numel() // _abi.SCATTER_DTYPE.itemsize,
+                                 int(scat_max_electron))
+        la = _abi.Launch(int(block), int(waves_per_simd), int(precision), 1)
+        s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        check(lib().mcd_forward(ctx, C.byref(self.cfg), self._tables, int(seed), int(config_id), int(first), int(n),
+                                els.data_ptr(), C.byref(sb) if sb is not None else None, counters.data_ptr(),
+                                C.byref(la), s), ctx)
+
+    def run(self, n, seed=1, first=0, config_id=0, scats=False, scat_max_electron=None, scat_capacity=None,
+            precision=_abi.PREC_FP32, block=0, waves_per_simd=0) -> RunResult:
+        """Run electrons [first, first+n) and return host results."""
+        dev = torch.device("cuda", self.device)
+        els = torch.empty(max(n, 1) * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+        rows = None
+        if scats:
+            if scat_max_electron is None:
+                scat_max_electron = first + n
+            if scat_capacity is None:
+                scat_capacity = max(1024, 4096 * min(n, max(0, scat_max_electron - first)))
+            rows = torch.empty(scat_capacity * _abi.SCATTER_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        self.launch(n, els, cnt, seed=seed, first=first, config_id=config_id, scat_rows=rows,
+                    scat_max_electron=scat_max_electron or 0, precision=precision, block=block,
+                    waves_per_simd=waves_per_simd)
+        torch.cuda.synchronize(dev)
+        counters = cnt.cpu().numpy().view(np.uint64)
+        e = els.cpu().numpy()[: n * _abi.ELECTRON_DTYPE.itemsize].view(_abi.ELECTRON_DTYPE)
+        srows = None
+        if rows is not None:
+            used = int(counters[_abi.CNT_SCAT_ROWS])
+            if used > scat_capacity:
+                raise RuntimeError(f"scatter-row buffer overflow: {used} rows requested, capacity {scat_capacity}")
+            srows = rows.cpu().numpy()[: used * _abi.SCATTER_DTYPE.itemsize].view(_abi.SCATTER_DTYPE)
+            srows = srows[np.lexsort((srows["step"], srows["electron"]))]
+        return RunResult(els=e, counters=counters, scats=srows)
+
+
+def grids_for(mat: Material, E0, Emin, Wc, tol_E, tol_W):
+    """The (Moller, outer-shell Gryzinski) grids of a DS configuration."""
+    return (moller_grid(mat, E0, Emin, Wc).build_grids(tol_E, tol_W),
+            gryz_grid(mat, E0, Emin).build_grids(tol_E, tol_W))
+
+
+__all__ = ["Transport", "RunResult", "context", "grids_for", "TableGrid"]
