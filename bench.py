@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Scatter events/s of the DS electron transport on one MI355X.
+
+Workload: plain DS mode, semi-infinite Cu (Z=29), E0 = 20 keV tracked to
+Emin = 1 keV, s_tilt = 70 deg, Wc = 50 eV, tol_E = tol_W = 1e-3, 2^22 electrons,
+Philox seed 1.  One step = one mcd_forward launch over the whole electron batch
+(tables built beforehand, outputs resident in HBM).  The channel-pick table and
+the W tables are built once outside the timed region.
+
+Prints one JSON line.  With --gpus N > 1 it exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mc-discrete_amd"))
+
+METRIC = "scatter events/s on 1 MI355X (DS, Cu 20 keV, 70°); backward/forward time ratio"
+# algorithmic FP32 operations per DS scatter event for Cu (add/sub/mul/div/cmp = 1,
+# FMA = 2, sqrt/rsqrt/log/sin = 1; Philox integer work excluded)
+F_OPS_CU_DS = 243
+PEAK_FP32_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--electrons", type=int, default=1 << 22)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of each CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--waves-per-simd", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_threads():
+    n = os.environ.get("OMP_NUM_THREADS")
+    return max(1, int(n)) if n and n.isdigit() else max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baselines(tr, seconds, seed):
+    """The FP64 C oracle (OpenMP) and the Python reference-path restatement on
+    bounded samples of the same workload, timed on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # noqa: E402  (the checker: CPU-baseline leg only)
+    import baseline_b1 as B1  # noqa: E402
+
+    gm, gg = tr.grids
+    tm = O.Table(gm.Es, gm.Ws, tr.cdf(0))
+    tg = O.Table(gg.Es, gg.Ws, tr.cdf(1))
+    threads = cpu_threads()
+    # C oracle: grow the sample until it takes about `seconds`
+    n, ev, dt = 1024, 0, 0.0
+    while True:
+        t = time.perf_counter()
+        _, _, ev = O.run(tr.cfg, tm, tg, seed, 0, n, nthreads=threads)
+        dt = time.perf_counter() - t
+        if dt >= seconds or n >= (1 << 22):
+            break
+        n = int(n * min(8.0, max(2.0, 1.2 * seconds / max(dt, 1e-3))))
+    c_base = {"value": ev / dt, "unit": "events/s", "cores": threads, "kind": "port",
+              "sample": f"FP64 C oracle (OpenMP), electrons 0..{n - 1} of the same workload, {ev} events in {dt:.1f} s"}
+    # Python reference-path restatement under multiprocessing (4 trajectories per job)
+    P = B1.params_from(tr.cfg, tm, tg, seed=seed)
+    n1 = max(4 * threads, int(seconds * threads * 10000 / 480))
+    ev1, dt1, w = B1.run_pool(P, n1, workers=threads)
+    b1 = {"value": ev1 / dt1, "unit": "events/s", "cores": w, "kind": "port",
+          "sample": f"Python/NumPy reference-path restatement, Pool({w}) x 4 trajectories per job, "
+                    f"electrons 0..{n1 - 1}, {ev1} events in {dt1:.1f} s"}
+    return c_base, b1
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md and not launched", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from mcd import _abi, engine
+
+    dev = torch.device("cuda", 0)
+    tr = engine.Transport("Cu", 20000.0, 1000.0, Wc=50.0, s_tilt=70.0, mode="DS", tol_E=1e-3, tol_W=1e-3)
+    n = a.electrons
+    els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    for _ in range(a.warmup):
+        tr.launch(n, els, cnt, seed=a.seed, block=a.block, waves_per_simd=a.waves_per_simd)
+    torch.cuda.synchronize(dev)
+    cnt.zero_()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        starts[k].record()
+        tr.launch(n, els, cnt, seed=a.seed, block=a.block, waves_per_simd=a.waves_per_simd)
+        ends[k].record()
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    kern_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / a.steps
+    c = cnt.cpu().numpy().view("uint64")
+    events = int(c[_abi.CNT_EVENTS])
+    value = events / wall
+    ev_per_launch = events / a.steps
+    achieved = ev_per_launch * F_OPS_CU_DS / (kern_ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "transport_ds_cu20_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (pencil beam, Philox seed 1)",
+        "config": {"workload": "DS transport, Cu bulk 20 keV -> 1 keV, s_tilt 70 deg, Wc 50 eV, "
+                               "tol_E = tol_W = 1e-3, 2^22 electrons per step, seed 1",
+                   "electrons_per_step": n, "parallelism": "single GPU"},
+        "electrons_per_s": n * a.steps / wall,
+        "events_per_electron": ev_per_launch / n,
+        "backscatter_yield": float(c[_abi.CNT_BKS]) / (n * a.steps),
+        "fp64_guard_picks_per_event": float(c[_abi.CNT_FP64_PICKS]) / max(events, 1),
+        "kernel_ms": kern_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "ops_per_event": F_OPS_CU_DS,
+                     "frac_vs_nonpacked_78_6": achieved / 78.6},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        c_base, b1 = cpu_baselines(tr, a.cpu_seconds, a.seed)
+        out["cpu_baseline"] = c_base
+        out["cpu_baseline_reference_path"] = b1
+        out["speedup_vs_cpu_baseline"] = value / c_base["value"]
+        out["speedup_vs_reference_path"] = value / b1["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

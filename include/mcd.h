@@ -180,10 +180,11 @@ enum {
 
 /* Launch tuning (0 = library default). */
 typedef struct mcd_launch {
-  int32_t block;        /* threads per workgroup */
-  int32_t waves_per_simd; /* resident waves per SIMD for the persistent grid */
-  int32_t precision;    /* MCD_PREC_FP32 transport (default) or MCD_PREC_FP64 (all-FP64 run) */
-  int32_t refill_period;/* steps between wavefront refills (1 = every step) */
+  int32_t block;          /* threads per workgroup, multiple of 64, at most 256 */
+  int32_t waves_per_simd; /* occupancy variant: resident wavefronts per SIMD the kernel's
+                             register budget guarantees (1 = compiler's choice, 4, 5, 6, 8) */
+  int32_t precision;      /* MCD_PREC_FP32 transport (default) or MCD_PREC_FP64 (all-FP64 run) */
+  int32_t reserved;
 } mcd_launch;
 
 typedef struct mcd_ctx mcd_ctx;

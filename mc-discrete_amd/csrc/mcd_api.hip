@@ -435,6 +435,13 @@ extern "C" int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap, i
   return MCD_OK;
 }
 
+// occupancy variant used when the caller does not choose one
+static int default_minw(const mcd_config& cfg) {
+  const char* s = getenv("MCD_MINW");
+  if (s) return atoi(s);
+  return cfg.mode == MCD_MODE_CONT ? 1 : 1;
+}
+
 static bool same_config(const mcd_config& a, const mcd_config& b) {
   return a.mat.Z == b.mat.Z && a.mat.atnd == b.mat.atnd && a.mat.fermi_e == b.mat.fermi_e && a.E0 == b.E0 &&
          a.Emin == b.Emin && a.Wc == b.Wc && a.mat.n_shells == b.mat.n_shells;
@@ -473,20 +480,17 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     R.row_max_electron = scats->max_electron;
     R.row_cursor = (unsigned long long*)&counters[MCD_CNT_SCAT_ROWS];
   }
-  int block = launch && launch->block > 0 ? launch->block : (precision == MCD_PREC_FP64 ? 256 : 256);
-  if (block % 64 || block > (precision == MCD_PREC_FP64 ? 256 : 512))
-    return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64 and at most %d", block,
-                   precision == MCD_PREC_FP64 ? 256 : 512);
-  int per_cu = max_blocks_per_cu(c, precision, block);
-  if (launch && launch->waves_per_simd > 0) {
-    const int cap = launch->waves_per_simd * 4 / (block / 64);
-    if (cap >= 1 && cap < per_cu) per_cu = cap;
-  }
+  int block = launch && launch->block > 0 ? launch->block : 256;
+  if (block % 64 || block > 256) return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64, at most 256", block);
+  int minw = launch && launch->waves_per_simd > 0 ? launch->waves_per_simd : default_minw(*cfg);
+  if (!valid_minw(minw)) return set_err(ctx, MCD_ERR_INVALID, "waves_per_simd %d not one of 1, 4, 5, 6, 8", minw);
+  int per_cu = max_blocks_per_cu(c, precision, minw, block);
   uint64_t want = (n + block - 1) / block;
   uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)per_cu;
   if (grid > want) grid = want;
   if (grid < 1) grid = 1;
   HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
-  HIP_TRY(ctx, launch_transport(c, R, precision, (int)grid, block, s));
+  HIP_TRY(ctx, launch_transport(c, R, precision, minw, (int)grid, block, s));
+  HIP_TRY(ctx, launch_count_outcomes(els, n, (unsigned long long*)counters, s));
   return MCD_OK;
 }

@@ -189,6 +189,46 @@ __device__ __noinline__ int d_pick(double E, double u, const DevCfg& c, double* 
   if (lo >= n) lo = n - 1;
   return lab[order[lo]];
 }
+#pragma clang fp contract(off)
+// The same exact pick with the channel count fixed at compile time: the sigma
+// set lives in registers and the stable descending sort is an unrolled
+// bubble network (adjacent exchanges only on strict '<', so ties keep the
+// insertion order as Python's stable sort does).
+template <int NCH>
+__device__ __forceinline__ int d_pick_fixed(double E, double u, const DevCfg& c, double& mfp) {
+  double v[NCH];
+  int id[NCH];
+  v[0] = d_sigma_ruth(E, c);
+  v[1] = d_sigma_moller(E, c);
+#pragma unroll
+  for (int s = 0; s < NCH - 3; ++s) v[2 + s] = d_sigma_gryz(E, c.Eb[s], c.ns[s]);
+  v[NCH - 1] = d_sigma_quinn(E, c);
+  double tot_ins = 0.;
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) { tot_ins += v[i]; id[i] = i; }
+  mfp = 1. / (c.atnd * tot_ins) * 1e14;
+#pragma unroll
+  for (int pass = 0; pass < NCH - 1; ++pass) {
+#pragma unroll
+    for (int i = 0; i < NCH - 1 - pass; ++i) {
+      const bool sw = v[i] < v[i + 1];
+      const double a = v[i], b = v[i + 1];
+      const int ia = id[i], ib = id[i + 1];
+      v[i] = sw ? b : a; v[i + 1] = sw ? a : b;
+      id[i] = sw ? ib : ia; id[i + 1] = sw ? ia : ib;
+    }
+  }
+  double acc = 0.;
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) { acc += v[i]; v[i] = acc; }
+  int cnt = 0;
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) cnt += (v[i] / acc < u) ? 1 : 0;
+  int pick = id[NCH - 1];
+#pragma unroll
+  for (int i = NCH - 1; i >= 0; --i) pick = (cnt == i) ? id[i] : pick;
+  return pick == 0 ? MCD_CH_RUTH : (pick == 1 ? MCD_CH_MOLLER : (pick == NCH - 1 ? MCD_CH_QUINN : MCD_CH_GRYZ0 + pick - 2));
+}
 #pragma clang fp contract(on)
 
 // ---------------------------------------------------------------------------
@@ -234,17 +274,20 @@ __device__ __forceinline__ void f_pick_eval(const float* __restrict__ nodes, con
     const float pB_k1 = b[j] - bm, pA_k1 = bp - b[j];
     o.pA = pA_k + f * (pA_k1 - pA_k);
     o.pB = pB_k + f * (pB_k1 - pB_k);
-    const float prev = j > 0 ? o.B[j - 1] : 0.f;
-    if (o.pB > o.pA) {
-      o.B[j] = prev + o.pB;
+    const bool bfirst = o.pB > o.pA;
+    const float pj = bfirst ? o.pB : o.pA;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const float prev = i > 0 ? o.B[i - 1] : 0.f;
+      o.B[i] = (i == j) ? prev + pj : o.B[i];
+    }
+    if (bfirst) {
       uint64_t perm = ((uint64_t)o.perm_hi << 32) | o.perm_lo;
       const uint64_t la = (perm >> (4 * j)) & 15u, lb = (perm >> (4 * (j + 1))) & 15u;
       perm &= ~((uint64_t)0xFFu << (4 * j));
       perm |= (lb << (4 * j)) | (la << (4 * (j + 1)));
       o.perm_lo = (uint32_t)perm;
       o.perm_hi = (uint32_t)(perm >> 32);
-    } else {
-      o.B[j] = prev + o.pA;
     }
   }
 }

@@ -18,8 +18,11 @@ struct RunArgs {
   unsigned long long* queue;
 };
 
-hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int grid, int block, hipStream_t s);
-int max_blocks_per_cu(const DevCfg& c, int precision, int block);
+hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int minw, int grid, int block,
+                            hipStream_t s);
+int max_blocks_per_cu(const DevCfg& c, int precision, int minw, int block);
+bool valid_minw(int w);
+hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsigned long long* counters, hipStream_t s);
 
 // tables
 hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, double Ef, int kind, double n_el,

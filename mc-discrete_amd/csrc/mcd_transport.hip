@@ -52,8 +52,10 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const
 
 // ---------------------------------------------------------------------------
 // FP32 transport (DS and continuous).
-template <int MODE, int NCH>
-__global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
+// MINW: minimum resident wavefronts per SIMD the register allocation must
+// allow (1 = compiler's choice); the autotuner picks it per configuration.
+template <int MODE, int NCH, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R) {
   extern __shared__ float lds[];
   if (MODE == MCD_MODE_DS) {
     const int nf = c.pick.n_nodes * c.pick.stride;
@@ -68,8 +70,7 @@ __global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
   uint32_t step = 0, flags = 0;
   F2 E = {0.f, 0.f};
   float p[3] = {0.f, 0.f, 0.f}, pc[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
-  // counters: events, electrons, bks, abs, trsm, long, fp64 picks
-  uint64_t cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t n_events = 0, n_fp64 = 0;
 
   while (true) {
     if (open) {
@@ -104,7 +105,6 @@ __global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
     // ---- one loop iteration of trajectory_DS / trajectory_cont_cl ----
     if ((int64_t)step >= c.max_scatt) {  // num_scatt > maxScatt -> 'long'
       write_rec(&R.els[e - R.first], E.hi, d, p, step, MCD_OUT_LONG, 0, flags);
-      cnt[1]++; cnt[5]++;
       active = false;
       continue;
     }
@@ -120,10 +120,10 @@ __global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
       ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp);
       if (ch < 0) {  // guard band: settle the pick in FP64
         double m64;
-        ch = d_pick((double)E.hi + (double)E.lo, (double)u0, c, &m64);
+        ch = d_pick_fixed<NCH>((double)E.hi + (double)E.lo, (double)u0, c, m64);
         mfp = (float)m64;
         sflags |= MCD_FLAG_FP64_PICK;
-        cnt[6]++;
+        n_fp64++;
       }
       l = -mfp * __logf(u1);
       if (ch == MCD_CH_RUTH) {  // Rutherford_halfPol, scattering.py:125-152
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
       s2 = a * u1 * inv;
       c2 = om * (1.f + a) * inv;
     }
-    cnt[0]++;
+    n_events++;
 
     // update_xyz (electron.py:64-84), Kahan-compensated position
     const float v0 = l * d[0] - pc[0], v1 = l * d[1] - pc[1], v2 = l * d[2] - pc[2];
@@ -205,10 +205,6 @@ __global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
       // saveOutcomes: pre-loss energy, pre-rotation direction, last interior position
       const float sp[3] = {p[0] - pc[0], p[1] - pc[1], p[2] - pc[2]};
       write_rec(&R.els[e - R.first], Epre.hi, d, sp, step + 1, o1, o2, flags);
-      cnt[1]++;
-      cnt[2] += (o1 == MCD_OUT_BKS);
-      cnt[3] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
-      cnt[4] += (o1 == MCD_OUT_TRSM);
     }
     const bool rec_row = R.rows != nullptr && e < R.row_max_electron;
     if (!done || rec_row) f_rotate(c2, s2, u_az, d, y);  // update_direction
@@ -228,7 +224,9 @@ __global__ __launch_bounds__(512) void k_transport_f32(DevCfg c, RunArgs R) {
     step++;
     if (done) active = false;
   }
-  wave_add_counters(R.counters, cnt, 7, lane);
+  const uint64_t cnt[2] = {n_events, n_fp64};
+  wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
+  wave_add_counters(R.counters + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -308,7 +306,7 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
   uint64_t e = 0;
   uint32_t step = 0, flags = 0;
   double E = 0., p[3] = {0., 0., 0.}, d[3] = {0., 0., 0.}, y[3] = {0., 0., 0.};
-  uint64_t cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t n_events = 0;
   while (true) {
     if (open) {
       uint64_t idle = __ballot(!active);
@@ -340,14 +338,13 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
       const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
       const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
       write_rec(&R.els[e - R.first], (float)E, df, pf, step, MCD_OUT_LONG, 0, flags);
-      cnt[1]++; cnt[5]++;
       active = false;
       continue;
     }
     const U4 r = philox10((uint32_t)e, c.config_id | (uint32_t)(e >> 32) << 16, step, 0u, k0, k1);
     const double Epre = E;
     const DStep o = d_step_physics<MODE>(c, E, r);
-    cnt[0]++;
+    n_events++;
     double np_[3];
     d_advance(p, o.l, d, np_);
     int o1 = MCD_OUT_NONE, o2 = MCD_OUT_NONE;
@@ -366,10 +363,6 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
       const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
       const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
       write_rec(&R.els[e - R.first], (float)Epre, df, pf, step + 1, o1, o2, flags);
-      cnt[1]++;
-      cnt[2] += (o1 == MCD_OUT_BKS);
-      cnt[3] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
-      cnt[4] += (o1 == MCD_OUT_TRSM);
     }
     const bool rec_row = R.rows != nullptr && e < R.row_max_electron;
     if (!done || rec_row) d_rotate(o.c2, o.hp, d, y);
@@ -390,19 +383,89 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
     step++;
     if (done) active = false;
   }
-  wave_add_counters(R.counters, cnt, 7, lane);
+  const uint64_t cnt = n_events;
+  wave_add_counters(R.counters + MCD_CNT_EVENTS, &cnt, 1, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Outcome counts from the electron records: electrons, bks, abs (both rows of
+// an exit-then-absorb electron count), trsm, long.
+__global__ void k_count_outcomes(const mcd_electron_rec* __restrict__ els, uint64_t n,
+                                 unsigned long long* __restrict__ counters) {
+  const int lane = threadIdx.x & 63;
+  uint64_t v[5] = {0, 0, 0, 0, 0};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const int o1 = els[i].outcome, o2 = els[i].outcome2;
+    v[0] += 1;
+    v[1] += (o1 == MCD_OUT_BKS);
+    v[2] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
+    v[3] += (o1 == MCD_OUT_TRSM);
+    v[4] += (o1 == MCD_OUT_LONG);
+  }
+  wave_add_counters(counters + MCD_CNT_ELECTRONS, v, 5, lane);
+}
+
+hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsigned long long* counters,
+                                 hipStream_t s) {
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_count_outcomes, dim3((unsigned)blocks), dim3(256), 0, s, els, n, counters);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // host-side launchers
 
-template <int MODE, int NCH>
+template <int MODE, int NCH, int MINW>
 static hipError_t launch_f32(const DevCfg& c, const RunArgs& R, int grid, int block, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_transport_f32<MODE, NCH>), dim3(grid), dim3(block), lds, s, c, R);
+  hipLaunchKernelGGL((k_transport_f32<MODE, NCH, MINW>), dim3(grid), dim3(block), lds, s, c, R);
   return hipGetLastError();
 }
 
-hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int grid, int block, hipStream_t s) {
+template <int MODE, int NCH, int MINW>
+static int occ_f32(int block, size_t lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MODE, NCH, MINW>, block, lds) != hipSuccess)
+    return 1;
+  return nb < 1 ? 1 : nb;
+}
+
+// dispatch over (channel count, occupancy variant)
+template <int MODE, int NCH>
+static hipError_t dispatch_w(int minw, const DevCfg& c, const RunArgs* R, int grid, int block, size_t lds,
+                             hipStream_t s, int* occ) {
+  switch (minw) {
+#define MCD_W(W)                                                   \
+  case W:                                                          \
+    if (occ) { *occ = occ_f32<MODE, NCH, W>(block, lds); return hipSuccess; } \
+    return launch_f32<MODE, NCH, W>(c, *R, grid, block, lds, s);
+    MCD_W(1) MCD_W(4) MCD_W(5) MCD_W(6) MCD_W(8)
+#undef MCD_W
+    default: return hipErrorInvalidValue;
+  }
+}
+
+static hipError_t dispatch_f32(int minw, const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s,
+                               int* occ) {
+  if (c.mode == MCD_MODE_CONT) return dispatch_w<MCD_MODE_CONT, 4>(minw, c, R, grid, block, 0, s, occ);
+  const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
+  switch (c.n_shells + 3) {
+    case 4: return dispatch_w<MCD_MODE_DS, 4>(minw, c, R, grid, block, lds, s, occ);
+    case 5: return dispatch_w<MCD_MODE_DS, 5>(minw, c, R, grid, block, lds, s, occ);
+    case 6: return dispatch_w<MCD_MODE_DS, 6>(minw, c, R, grid, block, lds, s, occ);
+    case 7: return dispatch_w<MCD_MODE_DS, 7>(minw, c, R, grid, block, lds, s, occ);
+    case 8: return dispatch_w<MCD_MODE_DS, 8>(minw, c, R, grid, block, lds, s, occ);
+    case 9: return dispatch_w<MCD_MODE_DS, 9>(minw, c, R, grid, block, lds, s, occ);
+    case 10: return dispatch_w<MCD_MODE_DS, 10>(minw, c, R, grid, block, lds, s, occ);
+    case 11: return dispatch_w<MCD_MODE_DS, 11>(minw, c, R, grid, block, lds, s, occ);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool valid_minw(int w) { return w == 1 || w == 4 || w == 5 || w == 6 || w == 8; }
+
+hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int minw, int grid, int block,
+                            hipStream_t s) {
   if (precision == MCD_PREC_FP64) {
     if (c.mode == MCD_MODE_DS)
       hipLaunchKernelGGL((k_transport_f64<MCD_MODE_DS>), dim3(grid), dim3(block), 0, s, c, R);
@@ -410,41 +473,20 @@ hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, in
       hipLaunchKernelGGL((k_transport_f64<MCD_MODE_CONT>), dim3(grid), dim3(block), 0, s, c, R);
     return hipGetLastError();
   }
-  if (c.mode == MCD_MODE_CONT) return launch_f32<MCD_MODE_CONT, 4>(c, R, grid, block, 0, s);
-  const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
-  switch (c.n_shells + 3) {
-    case 4: return launch_f32<MCD_MODE_DS, 4>(c, R, grid, block, lds, s);
-    case 5: return launch_f32<MCD_MODE_DS, 5>(c, R, grid, block, lds, s);
-    case 6: return launch_f32<MCD_MODE_DS, 6>(c, R, grid, block, lds, s);
-    case 7: return launch_f32<MCD_MODE_DS, 7>(c, R, grid, block, lds, s);
-    case 8: return launch_f32<MCD_MODE_DS, 8>(c, R, grid, block, lds, s);
-    case 9: return launch_f32<MCD_MODE_DS, 9>(c, R, grid, block, lds, s);
-    case 10: return launch_f32<MCD_MODE_DS, 10>(c, R, grid, block, lds, s);
-    case 11: return launch_f32<MCD_MODE_DS, 11>(c, R, grid, block, lds, s);
-    default: return hipErrorInvalidValue;
-  }
+  return dispatch_f32(minw, c, &R, grid, block, s, nullptr);
 }
 
-// occupancy query for the persistent grid
-int max_blocks_per_cu(const DevCfg& c, int precision, int block) {
-  int nb = 0;
-  hipError_t e;
+// resident blocks per CU for the persistent grid
+int max_blocks_per_cu(const DevCfg& c, int precision, int minw, int block) {
+  int nb = 1;
   if (precision == MCD_PREC_FP64) {
-    e = c.mode == MCD_MODE_DS
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f64<MCD_MODE_DS>, block, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f64<MCD_MODE_CONT>, block, 0);
-  } else if (c.mode == MCD_MODE_CONT) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_CONT, 4>, block, 0);
-  } else {
-    const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
-    switch (c.n_shells + 3) {
-      case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 6>, block, lds); break;
-      case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 7>, block, lds); break;
-      case 11: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 11>, block, lds); break;
-      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, 7>, block, lds); break;
-    }
+    hipError_t e = c.mode == MCD_MODE_DS
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f64<MCD_MODE_DS>, block, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f64<MCD_MODE_CONT>, block, 0);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    return nb;
   }
-  if (e != hipSuccess || nb < 1) nb = 1;
+  if (dispatch_f32(minw, c, nullptr, 0, block, 0, &nb) != hipSuccess || nb < 1) nb = 1;
   return nb;
 }
 

@@ -57,7 +57,7 @@ class ScatterBuf(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("waves_per_simd", C.c_int32), ("precision", C.c_int32),
-                ("refill_period", C.c_int32)]
+                ("reserved", C.c_int32)]
 
 
 # mcd_electron_rec (36 bytes)
