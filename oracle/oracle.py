@@ -59,7 +59,7 @@ def lib():
         L.oracle_sigma_moller.restype = d; L.oracle_sigma_moller.argtypes = [d, d, d]
         L.oracle_sigma_gryz.restype = d; L.oracle_sigma_gryz.argtypes = [d, d, d]
         L.oracle_sigma_quinn.restype = d; L.oracle_sigma_quinn.argtypes = [d, d, d, d]
-        L.oracle_sigma_set.restype = i; L.oracle_sigma_set.argtypes = [P(_abi.Config), P(d), P(i)]
+        L.oracle_sigma_set.restype = i; L.oracle_sigma_set.argtypes = [P(_abi.Config), d, P(d), P(i)]
         L.oracle_pick.restype = i; L.oracle_pick.argtypes = [P(d), i, d]
         L.oracle_table_row.restype = i; L.oracle_table_row.argtypes = [P(OracleTable), d]
         L.oracle_table_col.restype = i; L.oracle_table_col.argtypes = [P(OracleTable), i, d]

@@ -1,0 +1,256 @@
+"""GPU parity of the HIP transport against the FP64 oracle (same Philox stream,
+same table bytes), through the C-ABI (libmcd.so)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tables_ref as TR
+from conftest import CU20, SI10
+from mcd import _abi
+
+pytestmark = pytest.mark.gpu
+
+
+def transport(c, **kw):
+    from mcd import engine
+    return engine.Transport(c["material"], c["E0"], c["Emin"], Wc=c["Wc"], s_tilt=c["s_tilt"], tol_E=c["tol"],
+                            tol_W=c["tol"], **kw)
+
+
+def oracle_for(tr):
+    """Oracle tables holding the device-built CDF bytes."""
+    gm, gg = tr.grids
+    return O.Table(gm.Es, gm.Ws, tr.cdf(0)), O.Table(gg.Es, gg.Ws, tr.cdf(1))
+
+
+def identical(g, o):
+    return (g["outcome"] == o["outcome"]) & (g["outcome2"] == o["outcome2"]) & (g["n_scatt"] == o["n_scatt"])
+
+
+def check_values(g, o, same, rel_E=1e-5, dir_abs=1e-5, pos_rel=1e-5):
+    """Energies and directions of identical trajectories within tolerance; the
+    position error is measured against the trajectory's length scale
+    (mean free path x events), since a coordinate can sit near zero."""
+    E_err = np.abs(g["final_E"][same] - o["final_E"][same]) / o["final_E"][same]
+    assert E_err.max() <= rel_E, E_err.max()
+    d_err = np.abs(g["dir"][same] - o["dir"][same]).max()
+    assert d_err <= dir_abs, d_err
+    scale = np.maximum(np.linalg.norm(o["pos"][same], axis=1), 40.0 * np.sqrt(o["n_scatt"][same] + 1.0))
+    p_err = (np.linalg.norm(g["pos"][same] - o["pos"][same], axis=1) / scale).max()
+    assert p_err <= pos_rel, p_err
+    return E_err.max(), d_err, p_err
+
+
+@pytest.fixture(scope="module")
+def si10():
+    tr = transport(SI10)
+    tm, tg = oracle_for(tr)
+    return tr, tm, tg
+
+
+@pytest.fixture(scope="module")
+def cu20():
+    tr = transport(CU20)
+    tm, tg = oracle_for(tr)
+    return tr, tm, tg
+
+
+def test_device_tables_match_float32_restatement(si10, cu20):
+    """Device FP64 CDF rows (rounded once to float32) against the reference's
+    float32 arithmetic (probTables.py:415-462): same mask, values within the
+    float32 cumulative-sum error."""
+    for tr, _, _ in (si10, cu20):
+        m = tr.mat
+        gm, gg = tr.grids
+        ref = [TR.cdf_table(gm.Es, gm.Ws, m.fermi_e, "Moller", gm.n_el),
+               TR.cdf_table(gg.Es, gg.Ws, m.fermi_e, "Gryzinski", gg.n_el, gg.Eb)]
+        for which in (0, 1):
+            dev = tr.cdf(which)
+            assert np.array_equal(dev == 0, ref[which] == 0)
+            assert np.abs(dev - ref[which]).max() < 5e-6
+
+
+def test_device_fp32_table_build_matches_restatement(si10):
+    from mcd import engine
+    tr32 = engine.Transport("Si", 10000.0, 1000.0, Wc=100.0, s_tilt=70.0, tol_E=1e-2, tol_W=1e-2,
+                            table_precision=_abi.PREC_FP32, grids=si10[0].grids)
+    m = tr32.mat
+    gm, _ = tr32.grids
+    ref = TR.cdf_table(gm.Es, gm.Ws, m.fermi_e, "Moller", gm.n_el)
+    dev = tr32.cdf(0)
+    assert np.array_equal(dev == 0, ref == 0)
+    assert np.abs(dev - ref).max() < 1e-6
+
+
+@pytest.mark.parametrize("seed", [0, 9])
+def test_fp64_transport_matches_oracle_exactly(si10, seed):
+    tr, tm, tg = si10
+    n = 2000
+    o, _, ev = O.run(tr.cfg, tm, tg, seed, 0, n, nthreads=16)
+    r = tr.run(n, seed=seed, precision=_abi.PREC_FP64)
+    same = identical(r.els, o)
+    assert same.all(), np.flatnonzero(~same)[:10]
+    assert r.events == ev
+    check_values(r.els, o, same, rel_E=1e-7, dir_abs=1e-7, pos_rel=1e-7)
+
+
+@pytest.mark.parametrize("seed", [0, 9])
+def test_fp32_transport_matches_oracle(si10, seed):
+    tr, tm, tg = si10
+    n = 2000
+    o, _, ev = O.run(tr.cfg, tm, tg, seed, 0, n, nthreads=16)
+    r = tr.run(n, seed=seed)
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999, same.mean()
+    check_values(r.els, o, same)
+    c = r.counters
+    assert int(c[_abi.CNT_ELECTRONS]) == n
+    assert int(c[_abi.CNT_BKS]) == int((r.els["outcome"] == _abi.OUT_BKS).sum())
+
+
+def test_step_rows_match_oracle_trace(si10):
+    """Per-step scatter rows (type, W, angles, path, positions, energies,
+    directions) of the first 16 electrons against the oracle's trace."""
+    tr, tm, tg = si10
+    n, rec = 64, 16
+    _, tro, _ = O.run(tr.cfg, tm, tg, 4, 0, rec, trace_cap=100000)
+    r = tr.run(n, seed=4, scats=True, scat_max_electron=rec)
+    s = r.scats
+    assert len(s) == len(tro)
+    assert np.array_equal(s["electron"], tro["electron"]) and np.array_equal(s["step"], tro["step"])
+    assert np.array_equal(s["channel"], tro["channel"])
+    assert np.array_equal(s["E_loss"], tro["E_loss"].astype(np.float32))
+    assert np.allclose(s["E"], tro["E"], rtol=1e-6)
+    assert np.allclose(s["energy"], tro["energy"], rtol=1e-6)
+    assert np.allclose(s["pol"], tro["pol"], rtol=1e-5, atol=1e-6)
+    assert np.allclose(s["az"], tro["az"], rtol=1e-6)
+    assert np.allclose(s["pathl"], tro["pathl"], rtol=2e-5)
+    for k in ("dx", "dy", "dz"):
+        assert np.abs(s[k] - tro[k]).max() < 2e-5
+    assert np.array_equal(s["outcome"], tro["outcome"].astype(np.uint8))
+
+
+def test_headline_config_first_1e4_electrons(cu20):
+    """Cu 20 keV, 70 deg, Wc 50, tol 1e-3: the first 10^4 electrons of the
+    headline workload trajectory by trajectory."""
+    tr, tm, tg = cu20
+    n = 10000
+    o, _, ev = O.run(tr.cfg, tm, tg, 1, 0, n, nthreads=16)
+    r = tr.run(n, seed=1)
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999, same.mean()
+    check_values(r.els, o, same)
+    r64 = tr.run(n, seed=1, precision=_abi.PREC_FP64)
+    assert identical(r64.els, o).all()
+
+
+def test_headline_config_full_size_statistics(cu20):
+    """2^22 electrons: FP32 vs the all-FP64 GPU run of the same stream
+    (identical trajectories), and vs an independent FP64 sample (yield within
+    3 sigma, two-sample KS on BSE energy and exit angle, p > 0.01)."""
+    from scipy import stats
+    tr, _, _ = cu20
+    n = 1 << 22
+    r32 = tr.run(n, seed=1)
+    r64 = tr.run(n, seed=1, precision=_abi.PREC_FP64)
+    same = identical(r32.els, r64.els)
+    assert same.mean() >= 0.999, same.mean()
+    e32 = r32.els[same]
+    e64 = r64.els[same]
+    assert (np.abs(e32["final_E"] - e64["final_E"]) / e64["final_E"]).max() < 1e-5
+    ind = tr.run(n, seed=2, precision=_abi.PREC_FP64)
+    b32 = r32.els["outcome"] == _abi.OUT_BKS
+    bind = ind.els["outcome"] == _abi.OUT_BKS
+    y32, yind = b32.mean(), bind.mean()
+    sig = math.sqrt(y32 * (1 - y32) / n + yind * (1 - yind) / n)
+    assert abs(y32 - yind) < 3 * sig, (y32, yind, sig)
+    assert stats.ks_2samp(r32.els["final_E"][b32], ind.els["final_E"][bind]).pvalue > 0.01
+    ang32 = np.degrees(np.arccos(-r32.els["dir"][b32][:, 2]))
+    angind = np.degrees(np.arccos(-ind.els["dir"][bind][:, 2]))
+    assert stats.ks_2samp(ang32, angind).pvalue > 0.01
+
+
+@pytest.mark.parametrize("tilt", [0.0, 70.0])
+@pytest.mark.parametrize("bethe", ["classical", "JL", "explicit"])
+def test_continuous_mode_matches_oracle(tilt, bethe):
+    """Au 30 keV continuous mode with each Bethe law (trajectory_cont_cl,
+    singleScatter.py:205-239; stoppingPowers.py:112-215)."""
+    from mcd import engine
+    tr = engine.Transport("Au", 30000.0, 1000.0, s_tilt=tilt, mode="cont", bethe_model=bethe)
+    n = 2000
+    o, _, ev = O.run(tr.cfg, None, None, 2, 0, n, nthreads=16)
+    r64 = tr.run(n, seed=2, precision=_abi.PREC_FP64)
+    assert identical(r64.els, o).all()
+    r = tr.run(n, seed=2)
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999, same.mean()
+    check_values(r.els, o, same)
+
+
+def test_split_launches_equal_one_launch(cu20):
+    """Philox keys use the global electron id: two half-range launches give
+    bit-identical records to one launch (the sharding invariance a multi-GPU
+    split relies on)."""
+    tr = cu20[0]
+    n = 1 << 16
+    full = tr.run(n, seed=5)
+    a = tr.run(n // 2, seed=5)
+    b = tr.run(n // 2, seed=5, first=n // 2)
+    both = np.concatenate([a.els, b.els])
+    assert both.tobytes() == full.els.tobytes()
+
+
+def test_max_scatt_and_thin_film_outcomes():
+    """maxScatt -> 'long' (singleScatter.py:19-23) and a finite thickness ->
+    'trsm' (electron.py:72-81) against the oracle."""
+    from mcd import engine
+    tr = engine.Transport("Si", 10000.0, 1000.0, Wc=100.0, s_tilt=0.0, tol_E=1e-2, tol_W=1e-2, thickness=3000.0,
+                          max_scatt=80)
+    tm, tg = oracle_for(tr)
+    n = 2000
+    o, _, _ = O.run(tr.cfg, tm, tg, 6, 0, n, nthreads=16)
+    r = tr.run(n, seed=6)
+    for out in (_abi.OUT_LONG, _abi.OUT_TRSM, _abi.OUT_BKS):
+        assert (o["outcome"] == out).sum() > 50
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999
+
+
+def test_invalid_configuration_is_rejected_before_launch():
+    from mcd import engine
+    from mcd._lib import MCDError
+    with pytest.raises((MCDError, ValueError)):
+        engine.Transport("Au", 30000.0, 400.0, mode="cont", bethe_model="classical")  # S(Emin) < 0
+    with pytest.raises((MCDError, ValueError)):
+        engine.Transport("Cu", 1000.0, 2000.0, Wc=50.0)  # Emin > E0
+
+
+def test_scatter_buffer_overflow_is_reported(si10):
+    tr = si10[0]
+    with pytest.raises(RuntimeError, match="overflow"):
+        tr.run(64, seed=1, scats=True, scat_capacity=100)
+
+
+def test_reference_statistical_checks_on_gpu_output(si10):
+    """The reference's chi-square channel test and Watson U^2 polar-angle test
+    (tests/testThings.py:321-434) applied to GPU scatter rows."""
+    from scipy import stats
+    from test_oracle_algorithms import watson_u2
+    tr = si10[0]
+    r = tr.run(4000, seed=13, scats=True, scat_capacity=4000 * 600)
+    first = r.scats[r.scats["step"] == 0]
+    sig, lab = O.sigma_set(tr.cfg, 10000.0)
+    p = sig / sig.sum()
+    obs = np.array([(first["channel"] == l).sum() for l in lab], dtype=float)
+    exp = p * len(first)
+    keep = exp >= 5
+    chi2 = (((obs - exp) ** 2 / exp)[keep]).sum()
+    assert chi2 < stats.chi2.ppf(0.999, keep.sum() - 1)
+    ru = first[first["channel"] == _abi.CH_RUTH]
+    mc = np.degrees(np.arccos(2 * ru["pol"].astype(np.float64) - 1))[:200]
+    a = 3.4 * 14 ** 0.666667 / 10000.0
+    u = np.random.default_rng(3).random(200)
+    theory = np.degrees(np.arccos(1 - (2 * a * u) / (1 + a - u)))
+    assert watson_u2(theory, mc) < 0.152
