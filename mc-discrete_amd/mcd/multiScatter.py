@@ -1,0 +1,162 @@
+"""Drop-in batch workers of the reference (MC/tools/multiScatter.py) running
+the trajectories on the GPU.
+
+* ``genTables(inputPar)``   replaces probTables.genTables (probTables.py:541-614):
+  grids on the host, CDF rows on the GPU; only the Moller table and the
+  outermost-shell Gryzinski table are built, the ones the transport reads
+  (scattering.py:603,625).
+* ``multiTraj_DS(inputPar, numTraj, material, tables, thingsToSave)`` and
+  ``multiTraj_cont(inputPar, numTraj, material, thingsToSave, tables)`` keep the
+  reference's signatures (multiScatter.py:66,171) and return the same list of
+  per-electron ``{'els': {...}, 'scats': {...}}`` dicts.
+* ``scatterMultiEl_DS`` / ``scatterMultiEl_cont`` keep the legacy names
+  (multiScatter.py:15,118) and fill the given output mapping.
+* ``run_columns`` is the columnar fast path used by doScatter.py.
+
+Successive calls on one tables object continue the electron numbering, so the
+Philox streams (key = seed, counter = (electron, config, step, 0)) never repeat
+within a run; the reference instead re-seeded per job with pid*time
+(multiScatter.py:94), which repeats streams within a second.
+"""
+from __future__ import annotations
+
+from .engine import Transport
+from .fileTools import els_columns, per_electron_dicts, scats_columns, zipDict
+from .material import Material
+from .tables import gryz_grid, moller_grid
+
+
+class DeviceTables(dict):
+    """genTables() result: {'Moller': grid, 'Gryz': [outer-shell grid]} plus
+    the device transports built from them and the electron counter."""
+
+    def __init__(self, inputPar, material):
+        super().__init__()
+        self.inputPar = dict(inputPar)
+        self.material = material
+        self.next_electron = 0
+        self._transports = {}
+
+    def claim(self, n):
+        first = self.next_electron
+        self.next_electron += int(n)
+        return first
+
+    def transport(self, inputPar, material=None, mode=None):
+        p = dict(self.inputPar)
+        p.update(inputPar)
+        mode = mode or p.get("mode", "DS")
+        key = (mode, p.get("Bethe_model", "classical"), float(p.get("s_tilt", 0.0)),
+               float(p.get("thickness", float("inf"))), int(p.get("maxScatt", 0) or 0))
+        if key not in self._transports:
+            mat = _material(material or self.material, p)
+            grids = (self["Moller"], self["Gryz"][0]) if "Moller" in self else None
+            self._transports[key] = Transport(
+                mat, p["E0"], p["Emin"], Wc=p.get("Wc", 50.0), s_tilt=p.get("s_tilt", 0.0),
+                mode="DS" if "DS" in mode else "cont", bethe_model=p.get("Bethe_model", "classical"),
+                tol_E=p.get("tol_E", 1e-3), tol_W=p.get("tol_W", 1e-3),
+                thickness=p.get("thickness", float("inf")), max_scatt=int(p.get("maxScatt", 0) or 0),
+                grids=grids)
+        return self._transports[key]
+
+
+def _material(material, inputPar):
+    if isinstance(material, Material):
+        return material
+    if isinstance(material, str):
+        return Material(material)
+    return Material(inputPar["material"])
+
+
+def _check_elastic(inputPar):
+    el = inputPar.get("elastic", "Ruth_vanilla")
+    if el != "Ruth_vanilla":
+        raise NotImplementedError(f"elastic model {el!r}: only Ruth_vanilla is implemented "
+                                  "(Mott data and the Nigram/wDefl variants are not part of this build)")
+    if inputPar.get("diffMFP", False):
+        raise NotImplementedError("diffMFP: the diffraction channel is not part of this build")
+
+
+def genTables(inputPar):
+    """Tables for a run (probTables.genTables, probTables.py:541-614)."""
+    _check_elastic(inputPar)
+    mat = Material(inputPar["material"])
+    t = DeviceTables(inputPar, mat)
+    if "DS" in inputPar.get("mode", "DS"):
+        tol_E, tol_W = inputPar.get("tol_E", 1e-3), inputPar.get("tol_W", 1e-3)
+        t["Moller"] = moller_grid(mat, inputPar["E0"], inputPar["Emin"], inputPar["Wc"]).build_grids(tol_E, tol_W)
+        t["Gryz"] = [gryz_grid(mat, inputPar["E0"], inputPar["Emin"]).build_grids(tol_E, tol_W)]
+    return t
+
+
+def _tables_for(inputPar, tables):
+    if isinstance(tables, DeviceTables):
+        return tables
+    return genTables(inputPar)
+
+
+def _run(inputPar, numTraj, material, tables, thingsToSave, mode):
+    _check_elastic(inputPar)
+    tables = _tables_for(inputPar, tables)
+    tr = tables.transport(inputPar, material, mode)
+    first = tables.claim(numTraj)
+    want_scats = bool(thingsToSave.get("scat_output"))
+    r = tr.run(int(numTraj), seed=int(inputPar.get("seed", 0)), first=first, scats=want_scats)
+    return per_electron_dicts(r.els, r.scats, tr.mat, thingsToSave, first)
+
+
+def multiTraj_DS(inputPar, numTraj, material, tables, thingsToSave):
+    """numTraj DS trajectories (multiScatter.py:66-116)."""
+    return _run(inputPar, numTraj, material, tables, thingsToSave, "DS")
+
+
+def multiTraj_cont(inputPar, numTraj, material, thingsToSave, tables):
+    """numTraj continuous-slowing-down trajectories (multiScatter.py:171-229),
+    with the Bethe law of inputPar['Bethe_model'] (classical, JL or explicit)."""
+    return _run(inputPar, numTraj, material, tables, thingsToSave, "cont")
+
+
+def scatterMultiEl_DS(inputPar, tables, listParToSave, outputQ, count=0):
+    """Legacy entry point (multiScatter.py:15-61): inputPar['num_el']
+    trajectories; outputQ['els'] / outputQ['scats'] receive the zipped columns."""
+    res = multiTraj_DS(inputPar, inputPar["num_el"], inputPar.get("material"), tables, listParToSave)
+    els, scats = {}, {}
+    for r in res:
+        zipDict(els, r["els"])
+        zipDict(scats, r["scats"])
+    outputQ["els"], outputQ["scats"] = els, scats
+    return outputQ
+
+
+def scatterMultiEl_cont(inputPar, thingsToSave, output, num, count=0):
+    """Legacy entry point (multiScatter.py:118-168): `num` continuous-mode
+    trajectories; output['electrons'] / output['scatterings'] receive the
+    zipped columns."""
+    res = multiTraj_cont(inputPar, num, inputPar.get("material"), thingsToSave, None)
+    els, scats = {}, {}
+    for r in res:
+        zipDict(els, r["els"])
+        zipDict(scats, r["scats"])
+    output["electrons"], output["scatterings"] = els, scats
+    return output
+
+
+def run_columns(inputPar, n, tables=None, first=None, scat_max_electron=None, scat_capacity=None):
+    """Columnar run of n electrons: returns (els_columns, scats_columns or
+    None, RunResult).  scats are recorded for electrons with id <
+    scat_max_electron (default: all of them)."""
+    _check_elastic(inputPar)
+    tables = _tables_for(inputPar, tables)
+    tr = tables.transport(inputPar, None, inputPar.get("mode", "DS"))
+    first = tables.claim(n) if first is None else first
+    sc_want = list(inputPar.get("scatter_output", ()))
+    el_want = list(inputPar.get("electron_output", ("outcome", "final_E", "final_dir", "last_pos")))
+    r = tr.run(int(n), seed=int(inputPar.get("seed", 0)), first=first, scats=bool(sc_want),
+               scat_max_electron=scat_max_electron, scat_capacity=scat_capacity)
+    els = els_columns(r.els, el_want)
+    scats = scats_columns(r.scats, tr.mat, sc_want) if sc_want else None
+    return els, scats, r
+
+
+__all__ = ["genTables", "multiTraj_DS", "multiTraj_cont", "scatterMultiEl_DS", "scatterMultiEl_cont",
+           "run_columns", "DeviceTables"]
