@@ -226,6 +226,30 @@ int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, u
                 const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch,
                 void* stream);
 
+/* Batched launches (no reference counterpart: the reference runs one
+ * configuration per process, doScatter.py:155-252).  Item i uses Philox
+ * counter word config = i and electrons [0, n). */
+typedef struct mcd_batch_item {
+  const mcd_config* cfg;
+  const mcd_tables* tables;  /* DS: from mcd_tables_create for cfg; continuous: may be NULL */
+  uint64_t n;
+  mcd_electron_rec* els;     /* device, n records */
+} mcd_batch_item;
+
+typedef struct mcd_batch mcd_batch;
+
+/* Prepare a batch (uploads per-configuration constants and builds channel-pick
+ * tables of pick_nodes nodes (0 = 256) sized for per-wavefront LDS slices).
+ * The caller's tables and els buffers must outlive the batch. */
+int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n_items, int pick_nodes, mcd_batch** out);
+/* Run every item in one persistent launch.  counters: device array of
+ * n_items x MCD_NCOUNTERS uint64 (accumulated into).  use_graph != 0 records
+ * the launch sequence into a hipGraph once (per seed, counters and stream
+ * shape) and replays it.  Asynchronous on stream. */
+int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, const mcd_launch* launch,
+                  int use_graph, void* stream);
+int mcd_batch_destroy(mcd_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

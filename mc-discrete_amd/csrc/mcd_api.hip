@@ -224,7 +224,7 @@ static int upload_grid(mcd_ctx* ctx, const mcd_table_grid* gr, TableDev& td) {
   return MCD_OK;
 }
 
-static int pick_nodes_env() {
+static int pick_nodes_default() {
   const char* s = getenv("MCD_PICK_NODES");
   int n = s ? atoi(s) : 512;
   return n < 16 ? 16 : (n > 8192 ? 8192 : n);
@@ -245,11 +245,12 @@ static void set_meta(float* node, int n_ch, uint32_t swap, bool exact) {
 }
 
 // Build the LDS pick table for a DS configuration (see mcd_device.h).
-static int build_pick_table(mcd_ctx* ctx, mcd_tables* t) {
+static int build_pick_table(mcd_ctx* ctx, mcd_tables* t, int n_nodes) {
   const mcd_config& cfg = t->cfg;
   t->n_ch = cfg.mat.n_shells + 3;
   t->stride = ((t->n_ch + 2) + 3) / 4 * 4;
-  t->n_nodes = pick_nodes_env();
+  t->n_nodes = n_nodes;
+  if (t->nodes) { (void)hipFree(t->nodes); t->nodes = nullptr; }
   t->lnE0 = log(cfg.Emin);
   t->dln = (log(cfg.E0) - t->lnE0) / (double)(t->n_nodes - 1);
   t->delta = 0.f;
@@ -362,7 +363,7 @@ extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
         return set_err(ctx, MCD_ERR_DEVICE, "CDF build: %s", hipGetErrorString(e));
       }
     }
-    if ((rc = build_pick_table(ctx, t))) {
+    if ((rc = build_pick_table(ctx, t, pick_nodes_default()))) {
       mcd_tables_destroy(t);
       return rc;
     }
@@ -492,5 +493,176 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
   HIP_TRY(ctx, launch_transport(c, R, precision, minw, (int)grid, block, s));
   HIP_TRY(ctx, launch_count_outcomes(els, n, (unsigned long long*)counters, s));
+  return MCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches.
+struct mcd_batch {
+  int device;
+  int n_items;
+  int slice_floats;
+  std::vector<mcd_tables*> own_tables;  // per-item pick tables sized for the LDS slices
+  BatchItemDev* d_items;
+  unsigned long long* d_work;           // queues [n] then active [n]
+  hipStream_t capture_stream;
+  hipGraphExec_t graph;
+  uint64_t graph_seed;
+  uint64_t* graph_counters;
+  hipStream_t graph_stream;
+  int graph_block, graph_grid;
+};
+
+static double work_per_electron(const mcd_config& c) {
+  // events per electron, a rough model for load balancing only
+  const double dE = c.E0 - c.Emin;
+  if (c.mode == MCD_MODE_DS) return dE / 25.0;
+  const double E = c.E0, Z = c.mat.Z;
+  const double a = 3.4 * pow(Z, 0.666667) / E, Ek = E * 1e-3, r = (Ek + 511.) / (Ek + 1024.);
+  const double sig = 5.21e-21 * (Z * Z / (Ek * Ek)) * r * r * 4. * 3.141592653589793 / (a * (1. + a));
+  const double mfp = 1e14 / (c.mat.atnd * sig);
+  const double S = 2. * kPiEFour * c.mat.atnd * Z / E * log(1.166 * (E + c.mat.bethe_k * c.J) / c.J) * 1e-14;
+  return fmax(1.0, dE / fmax(mfp * S, 1e-3) * 0.6);
+}
+
+extern "C" int mcd_batch_destroy(mcd_batch* b) {
+  if (!b) return MCD_OK;
+  (void)hipSetDevice(b->device);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  for (mcd_tables* t : b->own_tables) mcd_tables_destroy(t);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_work) (void)hipFree(b->d_work);
+  delete b;
+  return MCD_OK;
+}
+
+extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n_items, int pick_nodes,
+                                mcd_batch** out) {
+  if (!ctx || !items || !out || n_items < 1 || n_items > 0xFFFF)
+    return set_err(ctx, MCD_ERR_INVALID, "mcd_batch_create: bad arguments (n_items=%d)", n_items);
+  *out = nullptr;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int nodes = pick_nodes > 0 ? pick_nodes : 256;
+  mcd_batch* b = new mcd_batch();
+  b->device = ctx->device;
+  b->n_items = n_items;
+  std::vector<BatchItemDev> h(n_items);
+  std::vector<double> work(n_items);
+  int slice = 0;
+  double tot = 0.;
+  for (int i = 0; i < n_items; ++i) {
+    const mcd_batch_item& it = items[i];
+    int rc = check_config(ctx, it.cfg);
+    if (rc) { mcd_batch_destroy(b); return rc; }
+    if (!it.els && it.n) { mcd_batch_destroy(b); return set_err(ctx, MCD_ERR_INVALID, "item %d: els is NULL", i); }
+    mcd_tables* own = nullptr;
+    if (it.cfg->mode == MCD_MODE_DS) {
+      if (!it.tables || !it.tables->has_w || !same_config(*it.cfg, it.tables->cfg)) {
+        mcd_batch_destroy(b);
+        return set_err(ctx, MCD_ERR_TABLE, "item %d: DS tables missing or built for another configuration", i);
+      }
+      // a private pick table sized for the per-wavefront LDS slice, sharing the W tables
+      own = new mcd_tables();
+      memset((void*)own, 0, sizeof(*own));
+      own->device = ctx->device;
+      own->cfg = *it.cfg;
+      own->has_w = false;
+      own->m = it.tables->m;
+      own->g = it.tables->g;
+      own->has_w = true;
+      rc = build_pick_table(ctx, own, nodes);
+      own->m = TableDev();  // W tables stay owned by the caller's object
+      own->g = TableDev();
+      own->has_w = false;
+      if (rc) { mcd_tables_destroy(own); mcd_batch_destroy(b); return rc; }
+      b->own_tables.push_back(own);
+      slice = slice > own->n_nodes * own->stride ? slice : own->n_nodes * own->stride;
+    }
+    DevCfg c = make_devcfg(*it.cfg, it.tables, (uint32_t)i);
+    if (own) {
+      c.pick.nodes = own->nodes;
+      c.pick.n_nodes = own->n_nodes;
+      c.pick.stride = own->stride;
+      c.pick.n_ch = own->n_ch;
+      c.pick.lnE0 = (float)own->lnE0;
+      c.pick.inv_dlnE = (float)(1.0 / own->dln);
+      c.pick.delta = own->delta;
+    }
+    memset(&h[i], 0, sizeof(BatchItemDev));
+    h[i].c = c;
+    h[i].R.first = 0;
+    h[i].R.n = it.n;
+    h[i].R.els = it.els;
+    work[i] = (double)it.n * work_per_electron(*it.cfg);
+    h[i].cost = (float)work_per_electron(*it.cfg);
+    tot += work[i];
+  }
+  double acc = 0.;
+  for (int i = 0; i < n_items; ++i) {
+    h[i].cum_lo = (float)(tot > 0 ? acc / tot : 0.);
+    acc += work[i];
+    h[i].cum_hi = (float)(tot > 0 ? acc / tot : 0.);
+  }
+  h[n_items - 1].cum_hi = 1.01f;
+  b->slice_floats = slice > 0 ? slice : 4;
+  if (hipMalloc(&b->d_items, n_items * sizeof(BatchItemDev)) != hipSuccess ||
+      hipMalloc(&b->d_work, 2 * (size_t)n_items * sizeof(unsigned long long)) != hipSuccess) {
+    mcd_batch_destroy(b);
+    return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the batch failed");
+  }
+  for (int i = 0; i < n_items; ++i) {
+    h[i].R.queue = b->d_work + i;
+    h[i].active = b->d_work + n_items + i;
+  }
+  hipError_t e = hipMemcpy(b->d_items, h.data(), n_items * sizeof(BatchItemDev), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { mcd_batch_destroy(b); return set_err(ctx, MCD_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e)); }
+  *out = b;
+  return MCD_OK;
+}
+
+static hipError_t batch_enqueue(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, int grid, int block,
+                                hipStream_t s) {
+  hipError_t e = launch_fill_u64(b->d_work, 0ull, 2 * b->n_items, s);
+  if (e == hipSuccess)
+    e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, (unsigned long long*)counters, grid, block, s);
+  if (e == hipSuccess) e = launch_count_outcomes_batched(b->d_items, b->n_items, (unsigned long long*)counters, s);
+  (void)ctx;
+  return e;
+}
+
+extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters,
+                             const mcd_launch* launch, int use_graph, void* stream) {
+  if (!ctx || !b || !counters) return set_err(ctx, MCD_ERR_INVALID, "mcd_batch_run: NULL argument");
+  if (launch && launch->precision == MCD_PREC_FP64)
+    return set_err(ctx, MCD_ERR_UNSUPPORTED, "batched launches run the FP32 transport");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int block = launch && launch->block > 0 ? launch->block : 256;
+  if (block % 64 || block > 256) return set_err(ctx, MCD_ERR_INVALID, "block %d invalid", block);
+  const int grid = ctx->n_cu * batched_blocks_per_cu(block, b->slice_floats);
+  hipStream_t s = (hipStream_t)stream;
+  if (!use_graph) {
+    HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, grid, block, s));
+    return MCD_OK;
+  }
+  if (!b->graph || b->graph_seed != seed || b->graph_counters != counters || b->graph_block != block ||
+      b->graph_grid != grid) {
+    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    if (!b->capture_stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+    HIP_TRY(ctx, hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = batch_enqueue(ctx, b, seed, counters, grid, block, b->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
+    HIP_TRY(ctx, e);
+    HIP_TRY(ctx, e2);
+    hipError_t e3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(ctx, e3);
+    b->graph_seed = seed;
+    b->graph_counters = counters;
+    b->graph_block = block;
+    b->graph_grid = grid;
+  }
+  HIP_TRY(ctx, hipGraphLaunch(b->graph, s));
   return MCD_OK;
 }

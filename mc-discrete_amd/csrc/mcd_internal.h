@@ -18,6 +18,21 @@ struct RunArgs {
   unsigned long long* queue;
 };
 
+// One configuration of a batched launch (device-resident).
+struct BatchItemDev {
+  DevCfg c;
+  RunArgs R;                   // els, n, queue (per item); seed/counters given at launch
+  float cost;                  // estimated work per electron (load balancing)
+  float cum_lo, cum_hi;        // work-fraction interval for the initial wave assignment
+  unsigned long long* active;  // wavefronts currently working on the item
+};
+
+hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
+                          unsigned long long* counters, int grid, int block, hipStream_t s);
+int batched_blocks_per_cu(int block, int slice_floats);
+hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
+                                         hipStream_t s);
+
 hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int minw, int grid, int block,
                             hipStream_t s);
 int max_blocks_per_cu(const DevCfg& c, int precision, int minw, int block);

@@ -52,16 +52,13 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const
 
 // ---------------------------------------------------------------------------
 // FP32 transport (DS and continuous).
-// MINW: minimum resident wavefronts per SIMD the register allocation must
-// allow (1 = compiler's choice); the autotuner picks it per configuration.
-template <int MODE, int NCH, int MINW>
-__global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R) {
-  extern __shared__ float lds[];
-  if (MODE == MCD_MODE_DS) {
-    const int nf = c.pick.n_nodes * c.pick.stride;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
-    __syncthreads();
-  }
+// The persistent loop of one wavefront over the electrons of one
+// configuration: refill idle lanes from the configuration's work counter,
+// step every active lane, write the record when a trajectory ends; returns
+// when the counter is exhausted and every lane is idle.
+template <int MODE, int NCH>
+__device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds, const RunArgs& R,
+                                               uint32_t& n_events, uint32_t& n_fp64) {
   const int lane = threadIdx.x & 63;
   const uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
 
@@ -70,7 +67,6 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
   uint32_t step = 0, flags = 0;
   F2 E = {0.f, 0.f};
   float p[3] = {0.f, 0.f, 0.f}, pc[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
-  uint32_t n_events = 0, n_fp64 = 0;
 
   while (true) {
     if (open) {
@@ -224,9 +220,145 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
     step++;
     if (done) active = false;
   }
+}
+
+// MINW: minimum resident wavefronts per SIMD the register allocation must
+// allow (1 = compiler's choice); the autotuner picks it per configuration.
+template <int MODE, int NCH, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R) {
+  extern __shared__ float lds[];
+  if (MODE == MCD_MODE_DS) {
+    const int nf = c.pick.n_nodes * c.pick.stride;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
+    __syncthreads();
+  }
+  uint32_t n_events = 0, n_fp64 = 0;
+  transport_loop<MODE, NCH>(c, lds, R, n_events, n_fp64);
+  const int lane = threadIdx.x & 63;
   const uint64_t cnt[2] = {n_events, n_fp64};
   wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
   wave_add_counters(R.counters + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launch: many configurations in one persistent grid.  A wavefront
+// works on one configuration at a time (its channel-pick table staged in the
+// wavefront's own LDS slice, its constants read with uniform loads), refills
+// from that configuration's counter and, once the counter is exhausted and its
+// lanes are idle, moves to the configuration with the most remaining work per
+// active wavefront.  Initial placement spreads wavefronts over configurations
+// in proportion to their estimated work.
+__device__ __forceinline__ int next_item(const BatchItemDev* __restrict__ items, int n_items, int lane) {
+  float best = 0.f;
+  int bi = -1;
+  for (int c = lane; c < n_items; c += 64) {
+    const unsigned long long q = __atomic_load_n(items[c].R.queue, __ATOMIC_RELAXED);
+    if (q < items[c].R.n) {
+      const unsigned long long act = __atomic_load_n(items[c].active, __ATOMIC_RELAXED);
+      const float rem = (float)(items[c].R.n - q) * items[c].cost / (float)(act + 1);
+      if (rem > best) { best = rem; bi = c; }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const float ob = __shfl_xor(best, off);
+    const int oi = __shfl_xor(bi, off);
+    if (ob > best || (ob == best && oi >= 0 && (bi < 0 || oi < bi))) { best = ob; bi = oi; }
+  }
+  return __builtin_amdgcn_readfirstlane(bi);
+}
+
+template <int MODE, int NCH>
+__device__ __forceinline__ void run_item(const BatchItemDev& it, const float* lds, uint64_t seed,
+                                         uint32_t& ne, uint32_t& nf) {
+  RunArgs R = it.R;
+  R.seed = seed;
+  transport_loop<MODE, NCH>(it.c, lds, R, ne, nf);
+}
+
+__global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* __restrict__ items, int n_items,
+                                                            int slice_floats, uint64_t seed,
+                                                            unsigned long long* __restrict__ counters) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  float* my = lds + (size_t)wib * slice_floats;
+  const int waves_total = gridDim.x * (blockDim.x >> 6);
+  const int wave_id = blockIdx.x * (blockDim.x >> 6) + wib;
+  const float f = ((float)wave_id + 0.5f) / (float)waves_total;
+  int ci = -1;
+  for (int c = 0; c < n_items; ++c)
+    if (f >= items[c].cum_lo && f < items[c].cum_hi) ci = c;
+  ci = __builtin_amdgcn_readfirstlane(ci);
+  if (ci < 0 || __atomic_load_n(items[ci].R.queue, __ATOMIC_RELAXED) >= items[ci].R.n) ci = next_item(items, n_items, lane);
+  while (ci >= 0) {
+    const BatchItemDev& it = items[ci];
+    if (lane == 0) atomicAdd(it.active, 1ull);
+    if (it.c.mode == MCD_MODE_DS) {
+      const int nfl = it.c.pick.n_nodes * it.c.pick.stride;
+      for (int i = lane; i < nfl; i += 64) my[i] = it.c.pick.nodes[i];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    uint32_t ne = 0, nf = 0;
+    if (it.c.mode == MCD_MODE_CONT) {
+      run_item<MCD_MODE_CONT, 4>(it, my, seed, ne, nf);
+    } else {
+      switch (it.c.n_shells + 3) {
+        case 4: run_item<MCD_MODE_DS, 4>(it, my, seed, ne, nf); break;
+        case 5: run_item<MCD_MODE_DS, 5>(it, my, seed, ne, nf); break;
+        case 6: run_item<MCD_MODE_DS, 6>(it, my, seed, ne, nf); break;
+        case 7: run_item<MCD_MODE_DS, 7>(it, my, seed, ne, nf); break;
+        case 8: run_item<MCD_MODE_DS, 8>(it, my, seed, ne, nf); break;
+        case 9: run_item<MCD_MODE_DS, 9>(it, my, seed, ne, nf); break;
+        case 10: run_item<MCD_MODE_DS, 10>(it, my, seed, ne, nf); break;
+        default: run_item<MCD_MODE_DS, 11>(it, my, seed, ne, nf); break;
+      }
+    }
+    const uint64_t cnt[2] = {ne, nf};
+    unsigned long long* ic = counters + (size_t)ci * MCD_NCOUNTERS;
+    wave_add_counters(ic + MCD_CNT_EVENTS, cnt, 1, lane);
+    wave_add_counters(ic + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
+    if (lane == 0) atomicSub(it.active, 1ull);
+    ci = next_item(items, n_items, lane);
+  }
+}
+
+hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
+                          unsigned long long* counters, int grid, int block, hipStream_t s) {
+  const size_t lds = (size_t)(block / 64) * slice_floats * sizeof(float);
+  hipLaunchKernelGGL(k_transport_batched, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
+                     counters);
+  return hipGetLastError();
+}
+
+int batched_blocks_per_cu(int block, int slice_floats) {
+  int nb = 0;
+  const size_t lds = (size_t)(block / 64) * slice_floats * sizeof(float);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_batched, block, lds) != hipSuccess || nb < 1)
+    nb = 1;
+  return nb;
+}
+
+__global__ void k_count_outcomes_batched(const BatchItemDev* __restrict__ items, unsigned long long* counters) {
+  const BatchItemDev& it = items[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  uint64_t v[5] = {0, 0, 0, 0, 0};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.R.n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int o1 = it.R.els[i].outcome, o2 = it.R.els[i].outcome2;
+    v[0] += 1;
+    v[1] += (o1 == MCD_OUT_BKS);
+    v[2] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
+    v[3] += (o1 == MCD_OUT_TRSM);
+    v[4] += (o1 == MCD_OUT_LONG);
+  }
+  wave_add_counters(counters + (size_t)blockIdx.y * MCD_NCOUNTERS + MCD_CNT_ELECTRONS, v, 5, lane);
+}
+
+hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
+                                         hipStream_t s) {
+  hipLaunchKernelGGL(k_count_outcomes_batched, dim3(64, n_items), dim3(256), 0, s, items, counters);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -55,6 +55,10 @@ class ScatterBuf(C.Structure):
     _fields_ = [("rows", C.c_void_p), ("capacity", C.c_uint64), ("max_electron", C.c_uint64)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("cfg", C.POINTER(Config)), ("tables", C.c_void_p), ("n", C.c_uint64), ("els", C.c_void_p)]
+
+
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("waves_per_simd", C.c_int32), ("precision", C.c_int32),
                 ("reserved", C.c_int32)]

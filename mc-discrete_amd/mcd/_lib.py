@@ -57,6 +57,12 @@ def lib():
     L.mcd_forward.restype = i32
     L.mcd_forward.argtypes = [vp, P(_abi.Config), vp, u64, u32, u64, u64, vp, P(_abi.ScatterBuf), vp,
                               P(_abi.Launch), vp]
+    L.mcd_batch_create.restype = i32
+    L.mcd_batch_create.argtypes = [vp, P(_abi.BatchItem), i32, i32, P(vp)]
+    L.mcd_batch_run.restype = i32
+    L.mcd_batch_run.argtypes = [vp, vp, u64, vp, P(_abi.Launch), i32, vp]
+    L.mcd_batch_destroy.restype = i32
+    L.mcd_batch_destroy.argtypes = [vp]
     if L.mcd_version() != 1:
         raise ImportError(f"libmcd.so ABI version {L.mcd_version()} does not match the bindings (1)")
     _lib = L

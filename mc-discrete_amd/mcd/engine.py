@@ -143,10 +143,60 @@ class Transport:
         return RunResult(els=e, counters=counters, scats=srows)
 
 
+class Batch:
+    """Many configurations in one persistent launch (mcd_batch_*).
+
+    items: list of (Transport, n_electrons).  Item i draws from Philox counter
+    word config = i, electrons 0..n-1, so results equal item-by-item runs with
+    config_id = i."""
+
+    def __init__(self, items, pick_nodes=0, device=0):
+        self.device = device
+        self.items = list(items)
+        dev = torch.device("cuda", device)
+        self.els = [torch.empty(max(n, 1) * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+                    for _, n in self.items]
+        self.counters = torch.zeros(len(self.items) * _abi.NCOUNTERS, dtype=torch.int64, device=dev)
+        arr = (_abi.BatchItem * len(self.items))()
+        for i, ((tr, n), e) in enumerate(zip(self.items, self.els)):
+            arr[i].cfg = C.pointer(tr.cfg)
+            arr[i].tables = tr._tables
+            arr[i].n = int(n)
+            arr[i].els = e.data_ptr()
+        self._arr = arr
+        self._h = C.c_void_p()
+        ctx = context(device)
+        check(lib().mcd_batch_create(ctx, arr, len(self.items), int(pick_nodes), C.byref(self._h)), ctx)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().mcd_batch_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def launch(self, seed=3, use_graph=True, block=0, stream=None):
+        ctx = context(self.device)
+        la = _abi.Launch(int(block), 0, _abi.PREC_FP32, 0)
+        s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        check(lib().mcd_batch_run(ctx, self._h, int(seed), self.counters.data_ptr(), C.byref(la), int(bool(use_graph)),
+                                  s), ctx)
+
+    def results(self):
+        torch.cuda.synchronize(self.device)
+        cnt = self.counters.cpu().numpy().view(np.uint64).reshape(len(self.items), _abi.NCOUNTERS)
+        out = []
+        for (tr, n), e, c in zip(self.items, self.els, cnt):
+            rec = e.cpu().numpy()[: n * _abi.ELECTRON_DTYPE.itemsize].view(_abi.ELECTRON_DTYPE)
+            out.append(RunResult(els=rec, counters=c.copy(), scats=None))
+        return out
+
+
 def grids_for(mat: Material, E0, Emin, Wc, tol_E, tol_W):
     """The (Moller, outer-shell Gryzinski) grids of a DS configuration."""
     return (moller_grid(mat, E0, Emin, Wc).build_grids(tol_E, tol_W),
             gryz_grid(mat, E0, Emin).build_grids(tol_E, tol_W))
 
 
-__all__ = ["Transport", "RunResult", "context", "grids_for", "TableGrid"]
+__all__ = ["Transport", "Batch", "RunResult", "context", "grids_for", "TableGrid"]
