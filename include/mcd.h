@@ -239,7 +239,8 @@ typedef struct mcd_batch_item {
 typedef struct mcd_batch mcd_batch;
 
 /* Prepare a batch (uploads per-configuration constants and builds channel-pick
- * tables of pick_nodes nodes (0 = 256) sized for per-wavefront LDS slices).
+ * tables of pick_nodes nodes; 0 sizes each table to an 8 KB per-wavefront LDS
+ * slice: 256 nodes for 3 shells, 170 for 4, 128 for 8).
  * The caller's tables and els buffers must outlive the batch. */
 int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n_items, int pick_nodes, mcd_batch** out);
 /* Run every item in one persistent launch.  counters: device array of

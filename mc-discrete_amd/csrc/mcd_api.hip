@@ -543,7 +543,8 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
     return set_err(ctx, MCD_ERR_INVALID, "mcd_batch_create: bad arguments (n_items=%d)", n_items);
   *out = nullptr;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  const int nodes = pick_nodes > 0 ? pick_nodes : 256;
+  // per-wavefront LDS slice: configuration copy + pick table within ~10 KB
+  const int pick_budget = 2048;  // floats
   mcd_batch* b = new mcd_batch();
   b->device = ctx->device;
   b->n_items = n_items;
@@ -571,6 +572,9 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
       own->m = it.tables->m;
       own->g = it.tables->g;
       own->has_w = true;
+      const int stride = ((it.cfg->mat.n_shells + 3 + 2) + 3) / 4 * 4;
+      int nodes = pick_nodes > 0 ? pick_nodes : pick_budget / stride;
+      if (nodes > 512) nodes = 512;
       rc = build_pick_table(ctx, own, nodes);
       own->m = TableDev();  // W tables stay owned by the caller's object
       own->g = TableDev();
@@ -605,7 +609,7 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
     h[i].cum_hi = (float)(tot > 0 ? acc / tot : 0.);
   }
   h[n_items - 1].cum_hi = 1.01f;
-  b->slice_floats = slice > 0 ? slice : 4;
+  b->slice_floats = batched_cfg_floats() + (slice > 0 ? slice : 4);
   if (hipMalloc(&b->d_items, n_items * sizeof(BatchItemDev)) != hipSuccess ||
       hipMalloc(&b->d_work, 2 * (size_t)n_items * sizeof(unsigned long long)) != hipSuccess) {
     mcd_batch_destroy(b);

@@ -30,6 +30,7 @@ struct BatchItemDev {
 hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
                           unsigned long long* counters, int grid, int block, hipStream_t s);
 int batched_blocks_per_cu(int block, int slice_floats);
+int batched_cfg_floats();
 hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
                                          hipStream_t s);
 

@@ -267,12 +267,16 @@ __device__ __forceinline__ int next_item(const BatchItemDev* __restrict__ items,
   return __builtin_amdgcn_readfirstlane(bi);
 }
 
+// The wavefront's LDS slice holds a copy of the configuration constants
+// followed by the channel-pick table, so the step loop reads both from LDS.
+constexpr int kCfgFloats = (int)((sizeof(DevCfg) + 15) / 16 * 4);
+
 template <int MODE, int NCH>
-__device__ __forceinline__ void run_item(const BatchItemDev& it, const float* lds, uint64_t seed,
+__device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, const float* pick, uint64_t seed,
                                          uint32_t& ne, uint32_t& nf) {
-  RunArgs R = it.R;
+  RunArgs R = itR;
   R.seed = seed;
-  transport_loop<MODE, NCH>(it.c, lds, R, ne, nf);
+  transport_loop<MODE, NCH>(c, pick, R, ne, nf);
 }
 
 __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* __restrict__ items, int n_items,
@@ -282,6 +286,7 @@ __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* _
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   float* my = lds + (size_t)wib * slice_floats;
+  float* pick = my + kCfgFloats;
   const int waves_total = gridDim.x * (blockDim.x >> 6);
   const int wave_id = blockIdx.x * (blockDim.x >> 6) + wib;
   const float f = ((float)wave_id + 0.5f) / (float)waves_total;
@@ -293,25 +298,31 @@ __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* _
   while (ci >= 0) {
     const BatchItemDev& it = items[ci];
     if (lane == 0) atomicAdd(it.active, 1ull);
-    if (it.c.mode == MCD_MODE_DS) {
+    // stage the configuration and its pick table in this wavefront's slice
+    const float* src = reinterpret_cast<const float*>(&it.c);
+    for (int i = lane; i < (int)(sizeof(DevCfg) / 4); i += 64) my[i] = src[i];
+    const int mode = it.c.mode;
+    const int nch = it.c.n_shells + 3;
+    if (mode == MCD_MODE_DS) {
       const int nfl = it.c.pick.n_nodes * it.c.pick.stride;
-      for (int i = lane; i < nfl; i += 64) my[i] = it.c.pick.nodes[i];
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
+      for (int i = lane; i < nfl; i += 64) pick[i] = it.c.pick.nodes[i];
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const DevCfg& c = *reinterpret_cast<const DevCfg*>(my);
     uint32_t ne = 0, nf = 0;
-    if (it.c.mode == MCD_MODE_CONT) {
-      run_item<MCD_MODE_CONT, 4>(it, my, seed, ne, nf);
+    if (mode == MCD_MODE_CONT) {
+      run_item<MCD_MODE_CONT, 4>(c, it.R, pick, seed, ne, nf);
     } else {
-      switch (it.c.n_shells + 3) {
-        case 4: run_item<MCD_MODE_DS, 4>(it, my, seed, ne, nf); break;
-        case 5: run_item<MCD_MODE_DS, 5>(it, my, seed, ne, nf); break;
-        case 6: run_item<MCD_MODE_DS, 6>(it, my, seed, ne, nf); break;
-        case 7: run_item<MCD_MODE_DS, 7>(it, my, seed, ne, nf); break;
-        case 8: run_item<MCD_MODE_DS, 8>(it, my, seed, ne, nf); break;
-        case 9: run_item<MCD_MODE_DS, 9>(it, my, seed, ne, nf); break;
-        case 10: run_item<MCD_MODE_DS, 10>(it, my, seed, ne, nf); break;
-        default: run_item<MCD_MODE_DS, 11>(it, my, seed, ne, nf); break;
+      switch (nch) {
+        case 4: run_item<MCD_MODE_DS, 4>(c, it.R, pick, seed, ne, nf); break;
+        case 5: run_item<MCD_MODE_DS, 5>(c, it.R, pick, seed, ne, nf); break;
+        case 6: run_item<MCD_MODE_DS, 6>(c, it.R, pick, seed, ne, nf); break;
+        case 7: run_item<MCD_MODE_DS, 7>(c, it.R, pick, seed, ne, nf); break;
+        case 8: run_item<MCD_MODE_DS, 8>(c, it.R, pick, seed, ne, nf); break;
+        case 9: run_item<MCD_MODE_DS, 9>(c, it.R, pick, seed, ne, nf); break;
+        case 10: run_item<MCD_MODE_DS, 10>(c, it.R, pick, seed, ne, nf); break;
+        default: run_item<MCD_MODE_DS, 11>(c, it.R, pick, seed, ne, nf); break;
       }
     }
     const uint64_t cnt[2] = {ne, nf};
@@ -320,8 +331,11 @@ __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* _
     wave_add_counters(ic + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
     if (lane == 0) atomicSub(it.active, 1ull);
     ci = next_item(items, n_items, lane);
+    __builtin_amdgcn_wave_barrier();
   }
 }
+
+int batched_cfg_floats() { return kCfgFloats; }
 
 hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
                           unsigned long long* counters, int grid, int block, hipStream_t s) {
