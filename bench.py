@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the timed steps, run one device-to-device copy of this size (the reference "
+                         "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE readings)")
     return ap.parse_args()
 
 
@@ -108,6 +111,13 @@ def main():
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
     kern_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / a.steps
+    if a.calibrate_copy_mib:
+        src = torch.empty(a.calibrate_copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize(dev)
+        dst.copy_(src)
+        torch.cuda.synchronize(dev)
+        del src, dst
     c = cnt.cpu().numpy().view("uint64")
     events = int(c[_abi.CNT_EVENTS])
     value = events / wall
