@@ -24,6 +24,7 @@ struct TableDev {
   float* Es = nullptr;
   float* Ws = nullptr;
   float* cdf = nullptr;
+  uint16_t* guide = nullptr;
   int nE = 0, nW = 0;
   float E_first = 0.f, inv_h = 0.f;
 };
@@ -170,7 +171,7 @@ static DevCfg make_devcfg(const mcd_config& cfg, const mcd_tables* t, uint32_t c
     const TableDev* src[2] = {&t->m, &t->g};
     DevTable* dst[2] = {&c.tm, &c.tg};
     for (int q = 0; q < 2; ++q) {
-      dst[q]->Es = src[q]->Es; dst[q]->Ws = src[q]->Ws; dst[q]->cdf = src[q]->cdf;
+      dst[q]->Es = src[q]->Es; dst[q]->Ws = src[q]->Ws; dst[q]->cdf = src[q]->cdf; dst[q]->guide = src[q]->guide;
       dst[q]->nE = src[q]->nE; dst[q]->nW = src[q]->nW;
       dst[q]->E_first = src[q]->E_first; dst[q]->inv_h = src[q]->inv_h;
     }
@@ -191,6 +192,7 @@ static void free_table(TableDev& t) {
   if (t.Es) (void)hipFree(t.Es);
   if (t.Ws) (void)hipFree(t.Ws);
   if (t.cdf) (void)hipFree(t.cdf);
+  if (t.guide) (void)hipFree(t.guide);
   t = TableDev();
 }
 
@@ -216,8 +218,10 @@ static int upload_grid(mcd_ctx* ctx, const mcd_table_grid* gr, TableDev& td) {
   td.E_first = gr->Es[0];
   td.inv_h = (float)((double)(gr->nE - 1) / ((double)gr->Es[gr->nE - 1] - (double)gr->Es[0]));
   const size_t ncdf = (size_t)gr->nE * (size_t)(gr->nW - 1);
+  if (gr->nW - 1 > 65535) return set_err(ctx, MCD_ERR_TABLE, "tables wider than 65535 loss bins are not supported");
   if (hipMalloc(&td.Es, gr->nE * sizeof(float)) != hipSuccess || hipMalloc(&td.Ws, gr->nW * sizeof(float)) != hipSuccess ||
-      hipMalloc(&td.cdf, ncdf * sizeof(float)) != hipSuccess)
+      hipMalloc(&td.cdf, ncdf * sizeof(float)) != hipSuccess ||
+      hipMalloc(&td.guide, (size_t)gr->nE * (kGuideQ + 1) * sizeof(uint16_t)) != hipSuccess)
     return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for a %d x %d table failed", gr->nE, gr->nW - 1);
   HIP_TRY(ctx, hipMemcpy(td.Es, gr->Es, gr->nE * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(td.Ws, gr->Ws, gr->nW * sizeof(float), hipMemcpyHostToDevice));
@@ -363,6 +367,13 @@ extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
         return set_err(ctx, MCD_ERR_DEVICE, "CDF build: %s", hipGetErrorString(e));
       }
     }
+    for (int q = 0; q < 2; ++q) {
+      hipError_t e = launch_build_guide(td[q]->cdf, td[q]->nE, td[q]->nW, td[q]->guide, 0);
+      if (e != hipSuccess) {
+        mcd_tables_destroy(t);
+        return set_err(ctx, MCD_ERR_DEVICE, "guide build: %s", hipGetErrorString(e));
+      }
+    }
     if ((rc = build_pick_table(ctx, t, pick_nodes_default()))) {
       mcd_tables_destroy(t);
       return rc;
@@ -399,6 +410,7 @@ extern "C" int mcd_tables_upload(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
     }
     hipError_t e = hipMemcpy(tmp, src[q], n * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = launch_mask(tmp, td[q]->cdf, n, 0);
+    if (e == hipSuccess) e = launch_build_guide(td[q]->cdf, td[q]->nE, td[q]->nW, td[q]->guide, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(tmp);
     if (e != hipSuccess) {

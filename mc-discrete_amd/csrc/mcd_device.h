@@ -30,10 +30,17 @@ constexpr int64_t kMaxStepsOff = 1ll << 24;  // step cap when maxScatt is off  /
 constexpr uint32_t kMetaNoSwap = 15u;
 constexpr uint32_t kMetaExact = 1u << 16;
 
+// Quantile guide of a CDF row: guide[row*(kGuideQ+1) + q] = first column whose
+// CDF >= q/kGuideQ, so the column of u lies in [guide[q], guide[q+1]] with
+// q = floor(u*kGuideQ); most lookups read one guide pair and at most a few
+// CDF entries instead of a full bisection.
+constexpr int kGuideQ = 128;
+
 struct DevTable {
   const float* Es;   // nE energies (float32 grid)
   const float* Ws;   // nW losses
   const float* cdf;  // nE x (nW-1), masked = kMasked
+  const uint16_t* guide;  // nE x (kGuideQ+1)
   int nE, nW;
   float E_first, inv_h;  // linspace index guess
 };
@@ -133,15 +140,15 @@ __device__ __forceinline__ double d_sigma_moller(double E, const DevCfg& c) {  /
 __device__ __forceinline__ double d_sigma_gryz(double E, double Esi, double nsi) {  // :213-242
   if (E <= Esi) return 0.;
   double U = E * 1. / Esi;
-  return nsi * kPiEFour * pow((U - 1.) / (U + 1.), 1.5) *
-         (1. + (2. * (1. - (1. / (2. * U)))) / 3. * log(2.7 + pow(U - 1., 0.5))) / (E * Esi);
+  const double q = (U - 1.) / (U + 1.);  // q^1.5 as q*sqrt(q): equal to pow to an ulp
+  return nsi * kPiEFour * (q * sqrt(q)) *
+         (1. + (2. * (1. - (1. / (2. * U)))) / 3. * log(2.7 + sqrt(U - 1.))) / (E * Esi);
 }
 __device__ __forceinline__ double d_sigma_quinn(double E, const DevCfg& c) {  // :247-284
   if (E <= c.Epl) return 0.;
   double E_Ef = E * 1. / c.Ef;
   double Epl_Ef = c.Epl * 1. / c.Ef;
-  double tot = c.Epl * log((pow(1. + Epl_Ef, 0.5) - 1.) / (pow(E_Ef, 0.5) - pow(E_Ef - Epl_Ef, 0.5))) /
-               (2. * kBohrR * E);
+  double tot = c.Epl * log((sqrt(1. + Epl_Ef) - 1.) / (sqrt(E_Ef) - sqrt(E_Ef - Epl_Ef))) / (2. * kBohrR * E);
   return (tot / c.atnd) * 1e4;
 }
 
@@ -190,47 +197,20 @@ __device__ __noinline__ int d_pick(double E, double u, const DevCfg& c, double* 
   return lab[order[lo]];
 }
 #pragma clang fp contract(off)
-// The same exact pick with the channel count fixed at compile time: the sigma
-// set lives in registers and the stable descending sort is an unrolled
-// bubble network (adjacent exchanges only on strict '<', so ties keep the
-// insertion order as Python's stable sort does).
-template <int NCH>
-__device__ __forceinline__ int d_pick_fixed(double E, double u, const DevCfg& c, double& mfp) {
-  double v[NCH];
-  int id[NCH];
-  v[0] = d_sigma_ruth(E, c);
-  v[1] = d_sigma_moller(E, c);
-#pragma unroll
-  for (int s = 0; s < NCH - 3; ++s) v[2 + s] = d_sigma_gryz(E, c.Eb[s], c.ns[s]);
-  v[NCH - 1] = d_sigma_quinn(E, c);
-  double tot_ins = 0.;
-#pragma unroll
-  for (int i = 0; i < NCH; ++i) { tot_ins += v[i]; id[i] = i; }
-  mfp = 1. / (c.atnd * tot_ins) * 1e14;
-#pragma unroll
-  for (int pass = 0; pass < NCH - 1; ++pass) {
-#pragma unroll
-    for (int i = 0; i < NCH - 1 - pass; ++i) {
-      const bool sw = v[i] < v[i + 1];
-      const double a = v[i], b = v[i + 1];
-      const int ia = id[i], ib = id[i + 1];
-      v[i] = sw ? b : a; v[i + 1] = sw ? a : b;
-      id[i] = sw ? ib : ia; id[i + 1] = sw ? ia : ib;
-    }
-  }
-  double acc = 0.;
-#pragma unroll
-  for (int i = 0; i < NCH; ++i) { acc += v[i]; v[i] = acc; }
-  int cnt = 0;
-#pragma unroll
-  for (int i = 0; i < NCH; ++i) cnt += (v[i] / acc < u) ? 1 : 0;
-  int pick = id[NCH - 1];
-#pragma unroll
-  for (int i = NCH - 1; i >= 0; --i) pick = (cnt == i) ? id[i] : pick;
-  return pick == 0 ? MCD_CH_RUTH : (pick == 1 ? MCD_CH_MOLLER : (pick == NCH - 1 ? MCD_CH_QUINN : MCD_CH_GRYZ0 + pick - 2));
+// Sigma of one channel label in FP64 (one formula per call keeps the guard
+// path's live registers to a handful of doubles).
+__device__ __forceinline__ double d_sigma_label(int l, double E, const DevCfg& c) {
+  if (l == MCD_CH_RUTH) return d_sigma_ruth(E, c);
+  if (l == MCD_CH_MOLLER) return d_sigma_moller(E, c);
+  if (l == MCD_CH_QUINN) return d_sigma_quinn(E, c);
+  const int s = l - MCD_CH_GRYZ0;
+  return d_sigma_gryz(E, c.Eb[s], c.ns[s]);
+}
+
+__device__ __forceinline__ int insertion_index(int l, int n) {
+  return l == MCD_CH_RUTH ? 0 : (l == MCD_CH_MOLLER ? 1 : (l == MCD_CH_QUINN ? n - 1 : 2 + l - MCD_CH_GRYZ0));
 }
 #pragma clang fp contract(on)
-
 // ---------------------------------------------------------------------------
 // Interpolated pick (FP32).  Node k holds the FP64 sorted cumulative
 // probabilities at E_k; within cell [k, k+1] the order is constant except in
@@ -300,8 +280,7 @@ __device__ __forceinline__ int perm_label(uint32_t lo, uint32_t hi, int pos) {
 // requires the FP64 pick.  mfp is always set.
 template <int NCH>
 __device__ __forceinline__ int f_pick_table(const float* __restrict__ nodes, const DevPick& P, float E, float u,
-                                            float& mfp) {
-  PickEval<NCH> o;
+                                            float& mfp, PickEval<NCH>& o) {
   f_pick_eval<NCH>(nodes, P, E, o);
   mfp = o.mfp;
   if (o.exact) return -1;
@@ -317,34 +296,87 @@ __device__ __forceinline__ int f_pick_table(const float* __restrict__ nodes, con
   return perm_label(o.perm_lo, o.perm_hi, cnt);
 }
 
+// Exact FP64 pick for a lane whose interpolated pick fell inside the guard
+// band.  The descending-sigma order is the cell's order (node labels), with
+// the swap pair of a crossing cell ordered by comparing its two sigmas in
+// FP64 (ties keep insertion order); in exact-only cells the order is found by
+// repeated selection of the largest remaining sigma.  Cumulative sums run in
+// that order, so probs[k] = cum_k / total equals pickFromSigmas' FP64 values
+// (scattering.py:48-56) and the first k with probs[k] >= u is its bisect_left.
+#pragma clang fp contract(off)
+template <int NCH>
+__device__ __forceinline__ int d_pick_ordered(double E, double u, const DevCfg& c, uint32_t perm_lo, uint32_t perm_hi,
+                                           int swap, bool exact) {
+  uint64_t perm = ((uint64_t)(perm_hi & 0xFFFu) << 32) | perm_lo;
+  if (exact) {
+    // selection order: position k gets the largest remaining sigma (lowest insertion index on ties)
+    uint32_t used = 0;
+    uint64_t out = 0;
+#pragma unroll 1
+    for (int k = 0; k < NCH; ++k) {
+      int best = -1;
+      double bv = -1.0;
+#pragma unroll 1
+      for (int q = 0; q < NCH; ++q) {
+        if (used & (1u << q)) continue;
+        const int l = q == 0 ? MCD_CH_RUTH : (q == 1 ? MCD_CH_MOLLER : (q == NCH - 1 ? MCD_CH_QUINN : MCD_CH_GRYZ0 + q - 2));
+        const double v = d_sigma_label(l, E, c);
+        if (v > bv) { bv = v; best = q; }
+      }
+      used |= 1u << best;
+      const int lb = best == 0 ? MCD_CH_RUTH : (best == 1 ? MCD_CH_MOLLER : (best == NCH - 1 ? MCD_CH_QUINN : MCD_CH_GRYZ0 + best - 2));
+      out |= (uint64_t)lb << (4 * k);
+    }
+    perm = out;
+  } else if (swap != (int)kMetaNoSwap) {
+    const int la = (int)((perm >> (4 * swap)) & 15u), lb = (int)((perm >> (4 * (swap + 1))) & 15u);
+    const double sa = d_sigma_label(la, E, c), sb = d_sigma_label(lb, E, c);
+    const bool b_first = sb > sa || (sb == sa && insertion_index(lb, NCH) < insertion_index(la, NCH));
+    if (b_first) {
+      perm &= ~((uint64_t)0xFFu << (4 * swap));
+      perm |= ((uint64_t)lb << (4 * swap)) | ((uint64_t)la << (4 * (swap + 1)));
+    }
+  }
+  double tot = 0.;
+#pragma unroll 1
+  for (int k = 0; k < NCH; ++k) tot += d_sigma_label((int)((perm >> (4 * k)) & 15u), E, c);
+  double acc = 0.;
+  int last = (int)((perm >> (4 * (NCH - 1))) & 15u);
+#pragma unroll 1
+  for (int k = 0; k < NCH; ++k) {
+    const int l = (int)((perm >> (4 * k)) & 15u);
+    acc += d_sigma_label(l, E, c);
+    if (acc / tot >= u) return l;
+  }
+  return last;
+}
+#pragma clang fp contract(on)
+
 // ---------------------------------------------------------------------------
 // Inverse-CDF table lookup (pickTable, scattering.py:63-94).
 // Row = first grid energy >= E (bisect_left), column = first CDF entry >= u.
 template <class EnergyLess>
 __device__ __forceinline__ int table_row(const DevTable& t, float Eguess, EnergyLess es_lt_E) {
+  // linspace guess, then one pair of grid energies settles bisect_left exactly
   int i = (int)ceilf((Eguess - t.E_first) * t.inv_h);
-  i = max(0, min(i, t.nE - 1));
-  // settle bisect_left exactly: Es[i-1] < E <= Es[i]
-  int it = 0;
-  while (it < 4 && i > 0 && !es_lt_E(t.Es[i - 1])) { --i; ++it; }
-  while (it < 8 && i < t.nE && es_lt_E(t.Es[i])) { ++i; ++it; }
-  bool ok = (i == 0 || es_lt_E(t.Es[i - 1])) && (i == t.nE || !es_lt_E(t.Es[i]));
-  if (!ok) {
-    int lo = 0, hi = t.nE;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (es_lt_E(t.Es[mid])) lo = mid + 1; else hi = mid;
-    }
-    i = lo;
+  i = max(1, min(i, t.nE - 1));
+  const float lo = t.Es[i - 1], hi = t.Es[i];
+  if (es_lt_E(lo) && !es_lt_E(hi)) return i;
+  int a = 0, b = t.nE;
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    if (es_lt_E(t.Es[mid])) a = mid + 1; else b = mid;
   }
-  return min(i, t.nE - 1);
+  return min(a, t.nE - 1);
 }
 
 __device__ __forceinline__ int table_col(const DevTable& t, int row, float u) {
   const float* r = t.cdf + (size_t)row * (size_t)(t.nW - 1);
-  int lo = 0, hi = t.nW - 1;
+  const int q = min((int)(u * (float)kGuideQ), kGuideQ - 1);
+  const uint16_t* g = t.guide + (size_t)row * (kGuideQ + 1) + q;
+  int lo = g[0], hi = g[1];
   while (lo < hi) {
-    int mid = (lo + hi) >> 1;
+    const int mid = (lo + hi) >> 1;
     if (r[mid] < u) lo = mid + 1; else hi = mid;
   }
   return lo;

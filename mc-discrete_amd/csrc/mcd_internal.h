@@ -43,6 +43,7 @@ hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsign
 // tables
 hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, double Ef, int kind, double n_el,
                             double Eb, int precision, float* cdf, hipStream_t s);
+hipError_t launch_build_guide(const float* cdf, int nE, int nW, uint16_t* guide, hipStream_t s);
 hipError_t launch_pick_nodes(const DevCfg& c, int n_nodes, double lnE0, double dln, float* nodes, int stride,
                              hipStream_t s);
 hipError_t launch_pick_check(const DevCfg& c, int n_checks, float* err, int* bad, hipStream_t s);

@@ -96,6 +96,28 @@ hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, do
   return hipGetLastError();
 }
 
+// Quantile guide: guide[row][q] = bisect_left(cdf_row, q / kGuideQ) (masked
+// entries hold kMasked, above every q).
+__global__ void k_build_guide(const float* __restrict__ cdf, int nE, int nW, uint16_t* __restrict__ guide) {
+  const int row = blockIdx.x;
+  const int nb = nW - 1;
+  const float* r = cdf + (size_t)row * nb;
+  for (int q = threadIdx.x; q <= kGuideQ; q += blockDim.x) {
+    const float v = (float)q / (float)kGuideQ;
+    int lo = 0, hi = nb;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (r[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    guide[(size_t)row * (kGuideQ + 1) + q] = (uint16_t)lo;
+  }
+}
+
+hipError_t launch_build_guide(const float* cdf, int nE, int nW, uint16_t* guide, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_guide, dim3(nE), dim3(64), 0, s, cdf, nE, nW, guide);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // pick-table nodes: FP64 sorted cumulative probabilities, ln mfp, labels.
 __global__ void k_pick_nodes(DevCfg c, int n_nodes, double lnE0, double dln, float* nodes, int stride) {

@@ -113,11 +113,21 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       const float u0 = u01f(r.x), u1 = u01f(r.y), u2 = u01f(r.z);
       u_az = u01f(r.w);
       float mfp;
-      ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp);
+      PickEval<NCH> pe;
+      ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp, pe);
       if (ch < 0) {  // guard band: settle the pick in FP64
-        double m64;
-        ch = d_pick_fixed<NCH>((double)E.hi + (double)E.lo, (double)u0, c, m64);
-        mfp = (float)m64;
+        // a crossing cell's swap was applied to pe.perm by the interpolation; undo it
+        // so d_pick_ordered sees node k's order and decides the pair exactly
+        uint32_t plo = pe.perm_lo, phi = pe.perm_hi;
+        if (pe.swap != (int)kMetaNoSwap && pe.pB > pe.pA) {
+          uint64_t pm = ((uint64_t)phi << 32) | plo;
+          const uint64_t la = (pm >> (4 * pe.swap)) & 15u, lb = (pm >> (4 * (pe.swap + 1))) & 15u;
+          pm &= ~((uint64_t)0xFFu << (4 * pe.swap));
+          pm |= (lb << (4 * pe.swap)) | (la << (4 * (pe.swap + 1)));
+          plo = (uint32_t)pm;
+          phi = (uint32_t)(pm >> 32);
+        }
+        ch = d_pick_ordered<NCH>((double)E.hi + (double)E.lo, (double)u0, c, plo, phi, pe.swap, pe.exact);
         sflags |= MCD_FLAG_FP64_PICK;
         n_fp64++;
       }
