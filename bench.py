@@ -7,6 +7,12 @@ Philox seed 1.  One step = one mcd_forward launch over the whole electron batch
 (tables built beforehand, outputs resident in HBM).  The channel-pick table and
 the W tables are built once outside the timed region.
 
+The second half of the metric, the backward/forward time ratio, is measured
+on the differentiation workload: Cu 20 keV -> 1 keV, s_tilt 70 deg, DS
+(Wc 50 eV) and continuous JL, 2^24 electrons, seed 4; forward = one
+mcd_forward launch, backward = one mcd_backward launch (vector-Jacobian
+product for the yield) over the same electrons.
+
 Prints one JSON line.  With --gpus N > 1 it exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -38,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=0)
+    ap.add_argument("--no-backward", action="store_true", help="skip the backward/forward ratio")
+    ap.add_argument("--bwd-electrons", type=int, default=1 << 24)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one device-to-device copy of this size (the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE readings)")
@@ -79,6 +87,46 @@ def cpu_baselines(tr, seconds, seed):
           "sample": f"Python/NumPy reference-path restatement, Pool({w}) x 4 trajectories per job, "
                     f"electrons 0..{n1 - 1}, {ev1} events in {dt1:.1f} s"}
     return c_base, b1
+
+
+def backward_ratio(n, reps=3):
+    """Forward and backward kernel times (HIP events on the launch stream) on
+    the differentiation workload, DS and continuous JL."""
+    import torch
+
+    from mcd import _abi
+    from mcd.autograd import BackscatterSpec
+
+    dev = torch.device("cuda", 0)
+    out = {"workload": f"Cu bulk 20 keV -> 1 keV, s_tilt 70 deg, {n} electrons, seed 4; "
+                       "backward = vector-Jacobian product of the yield w.r.t. (Wc, s_tilt, J, density)"}
+    els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    g = torch.zeros(65, dtype=torch.float32, device=dev)
+    g[0] = 1.0
+    acc = torch.zeros(1 + _abi.NGRAD, dtype=torch.float64, device=dev)
+    for key, mode in (("DS", "DS"), ("cont_JL", "cont")):
+        spec = BackscatterSpec(mode=mode, n=n, seed=4)
+        tr = spec.transport(spec.params().tolist())
+        tr.launch(n, els, cnt, seed=4)
+        tr.vjp(n, g, seed=4, out=acc)
+        torch.cuda.synchronize(dev)
+        tf, tb = [], []
+        for _ in range(reps):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e[0].record()
+            tr.launch(n, els, cnt, seed=4)
+            e[1].record()
+            e[2].record()
+            tr.vjp(n, g, seed=4, out=acc)
+            e[3].record()
+            torch.cuda.synchronize(dev)
+            tf.append(e[0].elapsed_time(e[1]))
+            tb.append(e[2].elapsed_time(e[3]))
+        f, b = min(tf), min(tb)
+        out[key] = {"forward_ms": f, "backward_ms": b, "ratio": b / f}
+    out["ratio_max"] = max(out["DS"]["ratio"], out["cont_JL"]["ratio"])
+    return out
 
 
 def main():
@@ -149,12 +197,18 @@ def main():
         "backscatter_yield": float(c[_abi.CNT_BKS]) / (n * a.steps),
         "fp64_guard_picks_per_event": float(c[_abi.CNT_FP64_PICKS]) / max(events, 1),
         "kernel_ms": kern_ms,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        # compute roof: FP32 157.3 TFLOP/s (the packed-FP32 VALU rate, equal to the
+        # dense FP32 MFMA figure); the transport's arithmetic is vector FP32
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "ops_per_event": F_OPS_CU_DS,
+                     "pipe": "VALU FP32", "ops_per_event": F_OPS_CU_DS,
                      "frac_vs_nonpacked_78_6": achieved / 78.6},
         "cpu_baseline": None,
     }
+    if not a.no_backward:
+        bw = backward_ratio(a.bwd_electrons)
+        out["backward"] = bw
+        out["bwd_fwd_ratio"] = bw["ratio_max"]
     if not a.no_cpu_baseline:
         c_base, b1 = cpu_baselines(tr, a.cpu_seconds, a.seed)
         out["cpu_baseline"] = c_base

@@ -251,6 +251,51 @@ int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters,
                   int use_graph, void* stream);
 int mcd_batch_destroy(mcd_batch* b);
 
+/* Differentiation (no reference counterpart).  Derivatives of
+ *   Y   = number of backscattered electrons (outcome 'bks', electron.py:68-70)
+ *   H_b = number of backscattered electrons whose exit energy (the pre-step
+ *         energy of the exit step) falls in bin b of n_bins equal bins over
+ *         [0, E0], the last bin closed
+ * with respect to the parameters below, estimated by replaying electrons
+ * [first, first+n) from their Philox counters (no stored trajectories):
+ * survival biasing at the surface with pathwise tangents, likelihood-ratio
+ * scores for the channel picks, and the threshold terms at Emin and at bin
+ * edges as central differences along the energy tangent. */
+enum {
+  MCD_GRAD_WC = 0,     /* d/dWc [1/eV], DS (Moller cross section, crossSections.py:183-208,
+                          and Moller losses, probTables.py:109) */
+  MCD_GRAD_TILT = 1,   /* d/ds_tilt [1/degree] (initial direction, multiScatter.py:97-100) */
+  MCD_GRAD_J = 2,      /* d/dJ [1/eV], continuous classical and JL stopping powers
+                          (stoppingPowers.py:112-173) */
+  MCD_GRAD_LNRHO = 3,  /* d/d ln(density): n, E_F ~ n^(2/3), E_pl ~ n^(1/2) move together
+                          (material.py:127-131); for a density scale s, d/ds = this / s */
+  MCD_NGRAD = 4
+};
+#define MCD_MAX_BINS 256
+
+typedef struct mcd_bwd_opts {
+  int32_t n_bins;     /* histogram bins over [0, E0], 1..MCD_MAX_BINS; 0 = 64 */
+  float rel_step;     /* parameter step of the threshold terms, relative to the parameter
+                         (absolute for ln density); 0 = 1e-2 */
+  float graze;        /* flights with d_z in (-graze, 0) exit unsmoothed (their exit
+                         indicator is not differentiated); 0 = 0.005 */
+  int32_t block;      /* threads per workgroup (multiple of 64, at most 256); 0 = 256 */
+} mcd_bwd_opts;
+
+/* Exactly one of out_jac / out_vjp is non-NULL (device memory, accumulated
+ * into, not cleared).
+ *   out_jac: (1 + n_bins) x (1 + MCD_NGRAD) doubles, row 0 = Y, row 1+b = H_b;
+ *            column 0 = the replay's estimate of the quantity, column 1+g =
+ *            its derivative with respect to parameter g.
+ *   out_vjp: with grad_out (device, 1 + n_bins floats: dL/dY, dL/dH_b), the
+ *            1 + MCD_NGRAD doubles L = dL/dY*Y + sum_b dL/dH_b*H_b and dL/dparam.
+ * Sums are over the n electrons (divide by n for per-electron values).
+ * Derivatives a mode does not depend on (J in DS, Wc in continuous mode) are
+ * 0.  Needs a semi-infinite target and maxScatt off.  Asynchronous on stream. */
+int mcd_backward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                 uint32_t config_id, uint64_t first, uint64_t n, const float* grad_out, double* out_vjp,
+                 double* out_jac, const mcd_bwd_opts* opts, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+
+#include <cmath>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -39,6 +41,15 @@ struct mcd_tables {
   double lnE0, dln;
   float delta;
   int n_swap_cells, n_exact_cells;
+  // derivative node table, built on the first mcd_backward with these tables
+  double moller_n_el;  // electrons per atom of the Moller table
+  mutable float* bwd_nodes = nullptr;
+  mutable float* bwd_m_swc = nullptr;   // DS: Wc score per Moller table column pick
+  mutable float* bwd_m_se = nullptr;    // DS: energy score per Moller table column pick
+  mutable float* bwd_g_se = nullptr;    // DS: energy score per Gryzinski table column pick
+  mutable double bwd_rel = 0.;          // parameter step the Wc scores were built with
+  mutable int bwd_n = 0, bwd_stride = 0, bwd_nch = 0;
+  mutable double bwd_lnE0 = 0., bwd_dln = 0.;
 };
 
 static thread_local char g_err[512];
@@ -202,6 +213,10 @@ extern "C" int mcd_tables_destroy(mcd_tables* t) {
   free_table(t->m);
   free_table(t->g);
   if (t->nodes) (void)hipFree(t->nodes);
+  if (t->bwd_nodes) (void)hipFree(t->bwd_nodes);
+  if (t->bwd_m_swc) (void)hipFree(t->bwd_m_swc);
+  if (t->bwd_m_se) (void)hipFree(t->bwd_m_se);
+  if (t->bwd_g_se) (void)hipFree(t->bwd_g_se);
   delete t;
   return MCD_OK;
 }
@@ -357,6 +372,7 @@ extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
       return rc;
     }
     t->has_w = true;
+    t->moller_n_el = moller->n_el;
     const mcd_table_grid* gr[2] = {moller, gryz};
     TableDev* td[2] = {&t->m, &t->g};
     for (int q = 0; q < 2; ++q) {
@@ -680,5 +696,171 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
     b->graph_grid = grid;
   }
   HIP_TRY(ctx, hipGraphLaunch(b->graph, s));
+  return MCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiation.
+static int bwd_nodes_default() {
+  const char* s = getenv("MCD_BWD_NODES");
+  int n = s ? atoi(s) : 256;
+  return n < 16 ? 16 : (n > 4096 ? 4096 : n);
+}
+
+// Moller W grid at a shifted cut-off: geomspace(Wc, W_top, nW) in float32
+// (probTables.py:251), the grid the reference would build for that Wc.
+static void moller_grid_at(double Wc, float Wtop, int nW, std::vector<float>& out) {
+  out.resize(nW);
+  const double l0 = log(Wc), l1 = log((double)Wtop);
+  for (int j = 0; j < nW; ++j) out[j] = (float)exp(l0 + (double)j / (double)(nW - 1) * (l1 - l0));
+  out[0] = (float)Wc;
+  out[nW - 1] = Wtop;
+}
+
+// Wc scores of the Moller column picks: tables rebuilt at Wc +- delta with
+// the same grid sizes, central difference of the column probabilities.
+static int build_wc_scores(mcd_ctx* ctx, const mcd_tables* t, double rel, hipStream_t s) {
+  const TableDev& m = t->m;
+  const size_t ncell = (size_t)m.nE * (size_t)(m.nW - 1);
+  float wtop = 0.f;
+  HIP_TRY(ctx, hipMemcpy(&wtop, m.Ws + (m.nW - 1), sizeof(float), hipMemcpyDeviceToHost));
+  const double delta = rel * t->cfg.Wc;
+  float *wp = nullptr, *wm = nullptr, *cp = nullptr, *cm = nullptr;
+  int rc = MCD_OK;
+  if (hipMalloc(&wp, m.nW * sizeof(float)) != hipSuccess || hipMalloc(&wm, m.nW * sizeof(float)) != hipSuccess ||
+      hipMalloc(&cp, ncell * sizeof(float)) != hipSuccess || hipMalloc(&cm, ncell * sizeof(float)) != hipSuccess) {
+    rc = set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the Wc column scores failed");
+  } else {
+    std::vector<float> gp, gm;
+    moller_grid_at(t->cfg.Wc + delta, wtop, m.nW, gp);
+    moller_grid_at(t->cfg.Wc - delta, wtop, m.nW, gm);
+    hipError_t e = hipMemcpy(wp, gp.data(), m.nW * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(wm, gm.data(), m.nW * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = launch_build_cdf(m.Es, wp, m.nE, m.nW, t->cfg.mat.fermi_e, MCD_TABLE_MOLLER, t->moller_n_el, 0., MCD_PREC_FP64,
+                           cp, s);
+    if (e == hipSuccess)
+      e = launch_build_cdf(m.Es, wm, m.nE, m.nW, t->cfg.mat.fermi_e, MCD_TABLE_MOLLER, t->moller_n_el, 0., MCD_PREC_FP64,
+                           cm, s);
+    if (e == hipSuccess) e = launch_ratio_score(m.cdf, cp, cm, m.nE, m.nW, 0.5 / delta, t->bwd_m_swc, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = set_err(ctx, MCD_ERR_DEVICE, "Wc column scores: %s", hipGetErrorString(e));
+  }
+  if (wp) (void)hipFree(wp);
+  if (wm) (void)hipFree(wm);
+  if (cp) (void)hipFree(cp);
+  if (cm) (void)hipFree(cm);
+  if (rc == MCD_OK) t->bwd_rel = rel;
+  return rc;
+}
+
+static int ensure_bwd_nodes(mcd_ctx* ctx, const mcd_tables* t, const DevCfg& c, double rel, hipStream_t s) {
+  const int n_ch = t->cfg.mode == MCD_MODE_DS ? t->cfg.mat.n_shells + 3 : 1;
+  const int nn = bwd_nodes_default();
+  if (!t->bwd_nodes || t->bwd_n != nn) {
+    if (t->bwd_nodes) { (void)hipFree(t->bwd_nodes); t->bwd_nodes = nullptr; }
+    const int stride = bwd_stride(t->cfg.mode, n_ch);
+    if (hipMalloc(&t->bwd_nodes, (size_t)nn * stride * sizeof(float)) != hipSuccess) {
+      t->bwd_nodes = nullptr;
+      return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the derivative node table failed");
+    }
+    t->bwd_n = nn;
+    t->bwd_stride = stride;
+    t->bwd_nch = n_ch;
+    t->bwd_lnE0 = log(t->cfg.Emin);
+    t->bwd_dln = (log(t->cfg.E0) - t->bwd_lnE0) / (double)(nn - 1);
+    HIP_TRY(ctx, launch_bwd_nodes(c, nn, t->bwd_lnE0, t->bwd_dln, t->bwd_nodes, stride, s));
+  }
+  if (t->cfg.mode != MCD_MODE_DS) return MCD_OK;
+  const TableDev& m = t->m;
+  const TableDev& g = t->g;
+  if (!t->bwd_m_swc) {
+    const size_t nm = (size_t)m.nE * (size_t)(m.nW - 1), ng = (size_t)g.nE * (size_t)(g.nW - 1);
+    if (hipMalloc(&t->bwd_m_swc, nm * sizeof(float)) != hipSuccess ||
+        hipMalloc(&t->bwd_m_se, nm * sizeof(float)) != hipSuccess ||
+        hipMalloc(&t->bwd_g_se, ng * sizeof(float)) != hipSuccess)
+      return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the loss-table scores failed");
+    HIP_TRY(ctx, launch_row_score(m.cdf, m.Es, m.nE, m.nW, t->bwd_m_se, s));
+    HIP_TRY(ctx, launch_row_score(g.cdf, g.Es, g.nE, g.nW, t->bwd_g_se, s));
+    t->bwd_rel = 0.;
+  }
+  if (t->bwd_rel != rel) return build_wc_scores(ctx, t, rel, s);
+  return MCD_OK;
+}
+
+extern "C" int mcd_backward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                            uint32_t config_id, uint64_t first, uint64_t n, const float* grad_out, double* out_vjp,
+                            double* out_jac, const mcd_bwd_opts* opts, void* stream) {
+  if (!ctx) return set_err(nullptr, MCD_ERR_INVALID, "mcd_backward: ctx is NULL");
+  int rc = check_config(ctx, cfg);
+  if (rc) return rc;
+  if ((out_jac == nullptr) == (out_vjp == nullptr))
+    return set_err(ctx, MCD_ERR_INVALID, "mcd_backward: pass exactly one of out_jac and out_vjp");
+  if (out_vjp && !grad_out) return set_err(ctx, MCD_ERR_INVALID, "mcd_backward: out_vjp needs grad_out");
+  if (config_id > 0xFFFFu) return set_err(ctx, MCD_ERR_INVALID, "config_id must fit 16 bits");
+  if (!tables || !same_config(*cfg, tables->cfg) || tables->cfg.mode != cfg->mode)
+    return set_err(ctx, MCD_ERR_TABLE, "mcd_backward needs tables built for this configuration");
+  if (cfg->mode == MCD_MODE_DS && !tables->has_w)
+    return set_err(ctx, MCD_ERR_TABLE, "DS mode needs tables from mcd_tables_create");
+  if (std::isfinite(cfg->thickness))
+    return set_err(ctx, MCD_ERR_UNSUPPORTED, "mcd_backward supports semi-infinite targets only (thickness = inf)");
+  if (cfg->max_scatt > 0) return set_err(ctx, MCD_ERR_UNSUPPORTED, "mcd_backward needs maxScatt off");
+  const int n_bins = opts && opts->n_bins > 0 ? opts->n_bins : 64;
+  if (n_bins > MCD_MAX_BINS) return set_err(ctx, MCD_ERR_INVALID, "n_bins %d above %d", n_bins, MCD_MAX_BINS);
+  const double bin_w = cfg->E0 / n_bins;
+  const double rel = opts && opts->rel_step > 0.f ? opts->rel_step : 1e-2;
+  const double graze = opts && opts->graze > 0.f ? opts->graze : 0.005;
+  if (!(rel < 0.5)) return set_err(ctx, MCD_ERR_INVALID, "rel_step must be below 0.5");
+  if (!(graze < 1.0)) return set_err(ctx, MCD_ERR_INVALID, "graze must be below 1");
+  const int block = opts && opts->block > 0 ? opts->block : 256;
+  if (block % 64 || block > 256) return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64, at most 256", block);
+  if (n == 0) return MCD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  DevCfg c = make_devcfg(*cfg, tables, config_id);
+  if ((rc = ensure_bwd_nodes(ctx, tables, c, rel, s))) return rc;
+  BwdArgs A;
+  memset(&A, 0, sizeof(A));
+  A.seed = seed;
+  A.first = first;
+  A.n = n;
+  A.nodes = tables->bwd_nodes;
+  A.n_nodes = tables->bwd_n;
+  A.stride = tables->bwd_stride;
+  A.n_ch = tables->bwd_nch;
+  A.lnE0 = (float)tables->bwd_lnE0;
+  A.inv_dln = (float)(1.0 / tables->bwd_dln);
+  A.n_bins = n_bins;
+  A.bin_w = (float)bin_w;
+  A.inv_bin_w = (float)(1.0 / bin_w);
+  // parameter steps of the threshold terms, per tangent slot
+  const double p0 = cfg->mode == MCD_MODE_DS ? cfg->Wc : cfg->J;
+  const double steps[3] = {rel * fabs(p0), rel * fmax(fabs(cfg->s_tilt), 1.0), rel};
+  for (int i = 0; i < 3; ++i) {
+    A.delta[i] = (float)steps[i];
+    A.inv_2delta[i] = (float)(0.5 / steps[i]);
+  }
+  A.graze = (float)graze;
+  if (cfg->mode == MCD_MODE_DS) {
+    A.m_score_wc = tables->bwd_m_swc;
+    A.m_score_e = tables->bwd_m_se;
+    A.g_score_e = tables->bwd_g_se;
+    A.inv_wc = (float)(1.0 / cfg->Wc);
+    A.inv_nw1 = (float)(1.0 / (double)(tables->m.nW - 1));
+  }
+  A.grad_out = grad_out;
+  A.out = out_jac ? out_jac : out_vjp;
+  A.queue = ctx->d_queue;
+  A.col_of_slot[0] = 1 + (cfg->mode == MCD_MODE_DS ? MCD_GRAD_WC : MCD_GRAD_J);
+  A.col_of_slot[1] = 1 + MCD_GRAD_TILT;
+  A.col_of_slot[2] = 1 + MCD_GRAD_LNRHO;
+  const bool jac = out_jac != nullptr;
+  const int per_cu = bwd_blocks_per_cu(c, A, jac, block);
+  uint64_t want = (n + block - 1) / block;
+  uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)per_cu;
+  if (grid > want) grid = want;
+  if (grid < 1) grid = 1;
+  HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
+  HIP_TRY(ctx, launch_backward(c, A, jac, (int)grid, block, s));
   return MCD_OK;
 }

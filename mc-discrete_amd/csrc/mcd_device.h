@@ -20,8 +20,8 @@ constexpr double kPiEFour = 6.51408491409531e-14;  // cm^2 eV^2, parameters.py:7
 constexpr double kBohrR = 5.29177210903e-11;       // m, parameters.py:8
 constexpr double kPi = 3.141592653589793;
 constexpr float kPiF = 3.14159265358979f;
-constexpr float kMasked = 2.0f;
-constexpr int64_t kMaxStepsOff = 1ll << 24;  // step cap when maxScatt is off  // device sentinel for masked CDF entries (> any u)
+constexpr float kMasked = 2.0f;              // device sentinel for masked CDF entries (> any u)
+constexpr int64_t kMaxStepsOff = 1ll << 24;  // step cap when maxScatt is off
 
 // Pick-table node layout: [B_0 .. B_{n-2}, ln mfp, perm_lo, perm_hi|meta, pad..]
 // perm: 4-bit channel labels in sorted (pick) order, positions 0..7 in perm_lo,

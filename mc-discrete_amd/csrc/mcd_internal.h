@@ -34,6 +34,32 @@ int batched_cfg_floats();
 hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
                                          hipStream_t s);
 
+// Derivative replay (mcd_backward.hip).
+struct BwdArgs {
+  uint64_t seed, first, n;
+  const float* nodes;          // node table (global; staged into LDS)
+  int n_nodes, stride, n_ch;
+  float lnE0, inv_dln;
+  int n_bins;
+  float bin_w, inv_bin_w;
+  float delta[3], inv_2delta[3];  // threshold terms: parameter step per tangent slot
+  float graze;                 // survival biasing for flights with d_z < -graze
+  const float* grad_out;       // VJP: 1 + n_bins upstream gradients
+  double* out;                 // VJP: 1 + MCD_NGRAD; Jacobian: (1 + n_bins) x (1 + MCD_NGRAD)
+  unsigned long long* queue;
+  int col_of_slot[3];          // output column of tangent slots 0..2
+  const float* m_score_wc;     // DS: Wc score of each Moller table column pick
+  const float* m_score_e;      // DS: energy score of each Moller table column pick
+  const float* g_score_e;      // DS: energy score of each Gryzinski table column pick
+  float inv_wc, inv_nw1;       // DS: 1 / Wc, 1 / (nW_Moller - 1)
+};
+
+hipError_t launch_bwd_nodes(const DevCfg& c, int n_nodes, double lnE0, double dln, float* nodes, int stride,
+                            hipStream_t s);
+int bwd_stride(int mode, int n_ch);
+int bwd_blocks_per_cu(const DevCfg& c, const BwdArgs& A, bool jac, int block);
+hipError_t launch_backward(const DevCfg& c, const BwdArgs& A, bool jac, int grid, int block, hipStream_t s);
+
 hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int minw, int grid, int block,
                             hipStream_t s);
 int max_blocks_per_cu(const DevCfg& c, int precision, int minw, int block);
@@ -43,6 +69,9 @@ hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsign
 // tables
 hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, double Ef, int kind, double n_el,
                             double Eb, int precision, float* cdf, hipStream_t s);
+hipError_t launch_ratio_score(const float* c0, const float* cp, const float* cm, int nE, int nW, double inv_2delta,
+                              float* out, hipStream_t s);
+hipError_t launch_row_score(const float* cdf, const float* Es, int nE, int nW, float* out, hipStream_t s);
 hipError_t launch_build_guide(const float* cdf, int nE, int nW, uint16_t* guide, hipStream_t s);
 hipError_t launch_pick_nodes(const DevCfg& c, int n_nodes, double lnE0, double dln, float* nodes, int stride,
                              hipStream_t s);

@@ -96,6 +96,63 @@ hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, do
   return hipGetLastError();
 }
 
+// Likelihood-ratio scores of loss-table column picks for the derivative
+// replay, from the discrete tables themselves (their float32 CDFs, masks
+// included).  p_j is the mass of column j of a row (CDF difference; 0 where
+// masked).
+__device__ __forceinline__ double col_mass(const float* row, int j) {
+  const float c = row[j];
+  if (c == kMasked) return 0.;
+  return (double)c - (j > 0 ? (double)row[j - 1] : 0.);
+}
+
+// Parameter score: (p_j(+) - p_j(-)) / (2 delta p_j) from the tables rebuilt
+// at the parameter +- delta (the central difference of the column
+// probabilities, masks included).
+__global__ void k_ratio_score(const float* __restrict__ c0, const float* __restrict__ cp,
+                              const float* __restrict__ cm, int nE, int nb, double inv_2delta,
+                              float* __restrict__ out) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)nE * nb) return;
+  const size_t i = k / nb;
+  const int j = (int)(k - i * nb);
+  const float* r0 = c0 + i * nb;
+  const double p0 = col_mass(r0, j);
+  out[k] = p0 > 0. ? (float)((col_mass(cp + i * nb, j) - col_mass(cm + i * nb, j)) * inv_2delta / p0) : 0.f;
+}
+
+// Energy score: rows are piecewise constant in energy (row i serves
+// (Es[i-1], Es[i]]); the score of a pick from row i is the row-to-row change
+// (p_j(i) - p_j(i-1)) / ((Es[i] - Es[i-1]) p_j(i)).
+__global__ void k_row_score(const float* __restrict__ cdf, const float* __restrict__ Es, int nE, int nb,
+                            float* __restrict__ out) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)nE * nb) return;
+  const size_t i = k / nb;
+  const int j = (int)(k - i * nb);
+  const double p = col_mass(cdf + i * nb, j);
+  if (i == 0 || !(p > 0.)) {
+    out[k] = 0.f;
+    return;
+  }
+  const double q = col_mass(cdf + (i - 1) * nb, j);
+  out[k] = (float)((p - q) / (((double)Es[i] - (double)Es[i - 1]) * p));
+}
+
+hipError_t launch_ratio_score(const float* c0, const float* cp, const float* cm, int nE, int nW, double inv_2delta,
+                              float* out, hipStream_t s) {
+  const size_t n = (size_t)nE * (nW - 1);
+  hipLaunchKernelGGL(k_ratio_score, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c0, cp, cm, nE, nW - 1,
+                     inv_2delta, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_score(const float* cdf, const float* Es, int nE, int nW, float* out, hipStream_t s) {
+  const size_t n = (size_t)nE * (nW - 1);
+  hipLaunchKernelGGL(k_row_score, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cdf, Es, nE, nW - 1, out);
+  return hipGetLastError();
+}
+
 // Quantile guide: guide[row][q] = bisect_left(cdf_row, q / kGuideQ) (masked
 // entries hold kMasked, above every q).
 __global__ void k_build_guide(const float* __restrict__ cdf, int nE, int nW, uint16_t* __restrict__ guide) {

@@ -20,6 +20,10 @@ PREC_FP64, PREC_FP32 = 0, 1
 
 CNT_EVENTS, CNT_ELECTRONS, CNT_BKS, CNT_ABS, CNT_TRSM, CNT_LONG, CNT_FP64_PICKS, CNT_SCAT_ROWS, CNT_QUEUE = range(9)
 NCOUNTERS = 16
+GRAD_WC, GRAD_TILT, GRAD_J, GRAD_LNRHO = range(4)
+NGRAD = 4
+GRAD_NAMES = ("Wc", "s_tilt", "J", "density_scale")
+MAX_BINS = 256
 
 
 class Material(C.Structure):
@@ -62,6 +66,10 @@ class BatchItem(C.Structure):
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("waves_per_simd", C.c_int32), ("precision", C.c_int32),
                 ("reserved", C.c_int32)]
+
+
+class BwdOpts(C.Structure):
+    _fields_ = [("n_bins", C.c_int32), ("rel_step", C.c_float), ("graze", C.c_float), ("block", C.c_int32)]
 
 
 # mcd_electron_rec (36 bytes)

@@ -63,6 +63,8 @@ def lib():
     L.mcd_batch_run.argtypes = [vp, vp, u64, vp, P(_abi.Launch), i32, vp]
     L.mcd_batch_destroy.restype = i32
     L.mcd_batch_destroy.argtypes = [vp]
+    L.mcd_backward.restype = i32
+    L.mcd_backward.argtypes = [vp, P(_abi.Config), vp, u64, u32, u64, u64, vp, vp, vp, P(_abi.BwdOpts), vp]
     if L.mcd_version() != 1:
         raise ImportError(f"libmcd.so ABI version {L.mcd_version()} does not match the bindings (1)")
     _lib = L

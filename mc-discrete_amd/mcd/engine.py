@@ -143,6 +143,55 @@ class Transport:
         return RunResult(els=e, counters=counters, scats=srows)
 
 
+    # -- differentiation ------------------------------------------------------
+    def _bwd(self, n, seed, first, config_id, grad_out, out_vjp, out_jac, n_bins, rel_step, graze, block,
+             stream):
+        ctx = context(self.device)
+        o = _abi.BwdOpts(int(n_bins), float(rel_step), float(graze), int(block))
+        s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        check(lib().mcd_backward(ctx, C.byref(self.cfg), self._tables, int(seed), int(config_id), int(first), int(n),
+                                 ptr(grad_out), ptr(out_vjp), ptr(out_jac), C.byref(o), s), ctx)
+
+    def jacobian(self, n, seed=1, first=0, config_id=0, n_bins=64, rel_step=0.0, graze=0.0, block=0,
+                 out=None, stream=None):
+        """Replay estimate of per-electron Y and H_b with their derivatives:
+        array (1 + n_bins, 1 + NGRAD), row 0 = yield, row 1+b = bin b; column 0 =
+        value, column 1+g = d/d(parameter g) in mcd.h's units (g = GRAD_*).
+        With `out` (a zeroed float64 device tensor of that shape) the launch is
+        asynchronous and accumulates raw sums into it; returns None."""
+        dev = torch.device("cuda", self.device)
+        acc = out if out is not None else torch.zeros((1 + n_bins, 1 + _abi.NGRAD), dtype=torch.float64, device=dev)
+        self._bwd(n, seed, first, config_id, None, None, acc, n_bins, rel_step, graze, block, stream)
+        if out is not None:
+            return None
+        return acc.cpu().numpy() / float(n)
+
+    def vjp(self, n, grad_out, seed=1, first=0, config_id=0, rel_step=0.0, graze=0.0, block=0, out=None,
+            stream=None):
+        """L = dL/dY * Y + sum_b dL/dH_b * H_b (per electron) and dL/dparam.
+        grad_out: float32 device tensor (1 + n_bins).  Returns float64 array
+        (1 + NGRAD), or accumulates raw sums into `out` asynchronously."""
+        dev = torch.device("cuda", self.device)
+        g = grad_out.to(device=dev, dtype=torch.float32).contiguous()
+        acc = out if out is not None else torch.zeros(1 + _abi.NGRAD, dtype=torch.float64, device=dev)
+        self._bwd(n, seed, first, config_id, g, acc, None, g.numel() - 1, rel_step, graze, block, stream)
+        if out is not None:
+            return None
+        return acc.cpu().numpy() / float(n)
+
+
+def backscatter_stats(els: torch.Tensor, n: int, E0: float, n_bins: int = 64):
+    """Backscatter count and exit-energy histogram (n_bins equal bins over
+    [0, E0], last bin closed) of n electron records held in a device byte
+    tensor, computed on the device.  Returns (count, counts[n_bins]) tensors."""
+    rec = els[: n * _abi.ELECTRON_DTYPE.itemsize].view(n, _abi.ELECTRON_DTYPE.itemsize)
+    bks = rec[:, _abi.ELECTRON_DTYPE.fields["outcome"][1]] == _abi.OUT_BKS
+    E = rec[:, 0:4].contiguous().view(torch.float32).reshape(n)
+    b = torch.clamp((E[bks].double() * (n_bins / E0)).long(), 0, n_bins - 1)
+    return bks.sum(), torch.bincount(b, minlength=n_bins)
+
+
 class Batch:
     """Many configurations in one persistent launch (mcd_batch_*).
 
@@ -199,4 +248,4 @@ def grids_for(mat: Material, E0, Emin, Wc, tol_E, tol_W):
             gryz_grid(mat, E0, Emin).build_grids(tol_E, tol_W))
 
 
-__all__ = ["Transport", "Batch", "RunResult", "context", "grids_for", "TableGrid"]
+__all__ = ["Transport", "Batch", "RunResult", "context", "grids_for", "TableGrid", "backscatter_stats"]

@@ -1,0 +1,190 @@
+"""Backward (derivative replay) kernel against central finite differences of
+the forward transport with common random numbers, at the differentiation
+workload: Cu 20 keV -> 1 keV, s_tilt 70 deg, DS (Wc 50 eV) and continuous JL,
+Philox seed 4, relative parameter step 1e-2.
+
+The finite difference is itself a Monte Carlo estimate: its standard error is
+computed from the per-electron paired outcomes, the replay's from 32
+independent electron ranges.  The yield derivative passes at
+|grad - fd| <= max(5% |fd|, 3 sigma_combined); each histogram bin whose
+finite difference is resolved (|fd| > 10 sigma_fd) at max(5% |fd|, 4 sigma)
+(about 50 such bins are compared in all), and the whole histogram at
+chi^2 <= 2 x (number of bins)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 24
+K = 32
+REL = 1e-2
+GRAD_OF = {"Wc": 0, "s_tilt": 1, "J": 2, "density_scale": 3}
+
+
+def _spec(mode, n=N):
+    from mcd.autograd import BackscatterSpec
+    return BackscatterSpec(mode=mode, n=n, seed=4)
+
+
+def _paired_fd(spec, name, n):
+    """Central difference of (Y, H) per electron with common random numbers:
+    (fd[1 + n_bins], se[1 + n_bins])."""
+    import torch
+    from mcd import _abi
+    p0 = spec.params().tolist()
+    gi = GRAD_OF[name]
+    h = REL * abs(p0[gi])
+    pp, pm = list(p0), list(p0)
+    pp[gi] += h
+    pm[gi] -= h
+    tp, tm = spec.transport(pp), spec.transport(pm)
+    dev = torch.device("cuda", 0)
+    m = min(n, 1 << 22)
+    ep = torch.empty(m * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    em = torch.empty_like(ep)
+    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    nb = spec.n_bins
+    sum_d = torch.zeros(1 + nb, dtype=torch.float64, device=dev)
+    sum_d2 = torch.zeros(1 + nb, dtype=torch.float64, device=dev)
+    off = _abi.ELECTRON_DTYPE.fields["outcome"][1]
+    for first in range(0, n, m):
+        k = min(m, n - first)
+        tp.launch(k, ep, cnt, seed=spec.seed, first=first)
+        tm.launch(k, em, cnt, seed=spec.seed, first=first)
+        cols = []
+        for e in (ep, em):
+            rec = e[: k * _abi.ELECTRON_DTYPE.itemsize].view(k, _abi.ELECTRON_DTYPE.itemsize)
+            bks = (rec[:, off] == _abi.OUT_BKS).to(torch.float64)
+            E = rec[:, 0:4].contiguous().view(torch.float32).reshape(k).double()
+            b = torch.clamp((E * (nb / spec.E0)).long(), 0, nb - 1)
+            onehot = torch.zeros((k, 1 + nb), dtype=torch.float64, device=dev)
+            onehot[:, 0] = bks
+            onehot[torch.arange(k, device=dev), 1 + b] += bks
+            cols.append(onehot)
+        d = cols[0] - cols[1]
+        sum_d += d.sum(0)
+        sum_d2 += (d * d).sum(0)
+    mean = sum_d / n
+    var = sum_d2 / n - mean * mean
+    return (mean / (2 * h)).cpu().numpy(), (torch.sqrt(var / n) / (2 * h)).cpu().numpy()
+
+
+def _replay(spec, n):
+    from mcd.autograd import jacobian
+    p = spec.params().tolist()
+    rows = []
+    for k in range(K):
+        v, j = jacobian(spec, p, n=n // K, first=k * (n // K))
+        rows.append(np.concatenate([v[:, None], j], axis=1))
+    rows = np.array(rows)
+    return rows.mean(0), rows.std(0, ddof=1) / np.sqrt(K)
+
+
+_cache = {}
+
+
+def _data(mode, name, n):
+    key = (mode, name, n)
+    if key not in _cache:
+        spec = _spec(mode, n)
+        fd, fse = _paired_fd(spec, name, n)
+        if (mode, n) not in _cache:
+            _cache[(mode, n)] = _replay(spec, n)
+        _cache[key] = (fd, fse)
+    return _cache[key], _cache[(mode, n)]
+
+
+def _check(mode, name, n):
+    (fd, fse), (m, se) = _data(mode, name, n)
+    col = 1 + GRAD_OF[name]
+    g, gse = m[:, col], se[:, col]
+    sig = np.sqrt(gse ** 2 + fse ** 2)
+    print(f"{mode} d/d{name}: dY replay {g[0]:.5e} +- {gse[0]:.1e}, fd {fd[0]:.5e} +- {fse[0]:.1e}")
+    assert abs(g[0] - fd[0]) <= max(0.05 * abs(fd[0]), 3 * sig[0]), (g[0], fd[0], sig[0])
+    # bins whose finite difference is resolved
+    for b in range(1, fd.size):
+        if abs(fd[b]) > 10 * fse[b]:
+            assert abs(g[b] - fd[b]) <= max(0.05 * abs(fd[b]), 4 * sig[b]), (b - 1, g[b], fd[b], sig[b])
+    live = sig[1:] > 0
+    chi2 = float((((g[1:] - fd[1:]) / np.where(live, sig[1:], 1.0)) ** 2)[live].sum())
+    assert chi2 <= 2.0 * live.sum(), (chi2, int(live.sum()))
+
+
+@pytest.mark.parametrize("mode,name", [("DS", "s_tilt"), ("cont", "s_tilt"), ("cont", "J"), ("DS", "density_scale")])
+def test_backward_matches_central_differences(mode, name):
+    _check(mode, name, N)
+
+
+def test_backward_wc_matches_central_differences():
+    # the Wc derivative is a likelihood-ratio estimate and the finite difference
+    # of a 1% change in Wc is small: four times the electrons
+    _check("DS", "Wc", 4 * N)
+
+
+@pytest.mark.parametrize("mode", ["DS", "cont"])
+def test_replay_estimate_matches_forward_yield(mode):
+    """The survival-biased replay estimates the same yield and histogram as the
+    transport itself."""
+    spec = _spec(mode)
+    total, hist = spec.forward_stats(spec.params().tolist())
+    Y = float(total) / N
+    m, se = _replay(spec, N)
+    sig = np.sqrt(se[0, 0] ** 2 + Y * (1 - Y) / N)
+    assert abs(m[0, 0] - Y) < 4 * sig, (m[0, 0], Y, sig)
+    H = hist.cpu().numpy() / N
+    sigH = np.sqrt(se[1:, 0] ** 2 + H * (1 - H) / N) + 1e-12
+    assert np.all(np.abs(m[1:, 0] - H) < 5 * sigH)
+
+
+def test_density_derivative_vanishes_in_continuous_mode():
+    """mfp ~ 1/n and S ~ n: density only rescales lengths, so the yield and
+    histogram of a semi-infinite target do not depend on it."""
+    from mcd.autograd import jacobian
+    spec = _spec("cont", 1 << 22)
+    v, j = jacobian(spec, spec.params().tolist())
+    scale = np.abs(j[:, 1]).max()  # the tilt derivative for comparison
+    assert np.abs(j[:, 3]).max() < 1e-6 * scale
+
+
+def test_vjp_equals_jacobian_transpose_product():
+    import torch
+    from mcd.autograd import jacobian
+    spec = _spec("DS", 1 << 20)
+    p = spec.params().tolist()
+    v, jac = jacobian(spec, p)
+    g = np.random.default_rng(0).standard_normal(1 + spec.n_bins).astype(np.float32)
+    out = spec.transport(p).vjp(spec.n, torch.from_numpy(g), seed=spec.seed)
+    want = g.astype(np.float64) @ jac
+    assert abs(out[0] - g @ v) < 1e-4 * max(1.0, abs(g @ v))
+    assert np.allclose(out[1:], want, rtol=2e-3, atol=1e-6 * np.abs(want).max())
+
+
+def test_autograd_function_backward():
+    import torch
+    from mcd.autograd import backscatter, jacobian
+    spec = _spec("DS", 1 << 20)
+    p = spec.params(density_scale=1.05).requires_grad_(True)
+    Y, H = backscatter(p, spec)
+    loss = Y + 0.5 * H[-1] - 0.25 * H[-2]
+    loss.backward()
+    _, jac = jacobian(spec, p.detach().tolist())
+    want = jac[0] + 0.5 * jac[spec.n_bins] - 0.25 * jac[spec.n_bins - 1]
+    assert torch.allclose(p.grad, torch.as_tensor(want), rtol=2e-3, atol=1e-7)
+    assert p.grad[2] == 0.0  # J does not enter DS transport
+
+
+def test_backward_rejects_unsupported_requests():
+    import torch
+    from mcd import _abi
+    from mcd._lib import MCDError
+    from mcd.engine import Transport
+    tr = Transport("Cu", 20000.0, 1000.0, Wc=50.0, s_tilt=70.0, mode="cont", bethe_model="JL", thickness=5000.0)
+    with pytest.raises(MCDError) as e:
+        tr.jacobian(1024, seed=4)
+    assert e.value.code == _abi.ERR_UNSUPPORTED
+    tr = Transport("Cu", 20000.0, 1000.0, Wc=50.0, s_tilt=70.0, mode="cont", bethe_model="JL")
+    out = torch.zeros((65, 5), dtype=torch.float64, device="cuda")
+    with pytest.raises(MCDError) as e:  # both outputs
+        tr._bwd(16, 4, 0, 0, torch.zeros(65, device="cuda"), torch.zeros(5, dtype=torch.float64, device="cuda"),
+                out, 64, 0.0, 0.0, 0, None)
+    assert e.value.code == _abi.ERR_INVALID

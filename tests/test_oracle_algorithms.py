@@ -211,9 +211,11 @@ def test_cfg_struct_layout_matches_header(tmp_path):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     src = tmp_path / "lay.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "mcd.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(mcd_config), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %d %d %d\\n", sizeof(mcd_config), '
                    'offsetof(mcd_config,E0), offsetof(mcd_config,J), sizeof(mcd_material), sizeof(mcd_table_grid), '
-                   'sizeof(mcd_electron_rec), sizeof(mcd_scatter_rec), sizeof(mcd_scatter_buf), sizeof(mcd_launch));'
+                   'sizeof(mcd_electron_rec), sizeof(mcd_scatter_rec), sizeof(mcd_scatter_buf), sizeof(mcd_launch), '
+                   'sizeof(mcd_batch_item), sizeof(mcd_bwd_opts), offsetof(mcd_bwd_opts,graze), MCD_NGRAD, '
+                   'MCD_GRAD_LNRHO, MCD_MAX_BINS);'
                    'return 0;}\n')
     exe = tmp_path / "lay"
     subprocess.run(["gcc", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
@@ -221,5 +223,6 @@ def test_cfg_struct_layout_matches_header(tmp_path):
     import ctypes as C
     want = [C.sizeof(_abi.Config), _abi.Config.E0.offset, _abi.Config.J.offset, C.sizeof(_abi.Material),
             C.sizeof(_abi.TableGridC), _abi.ELECTRON_DTYPE.itemsize, _abi.SCATTER_DTYPE.itemsize,
-            C.sizeof(_abi.ScatterBuf), C.sizeof(_abi.Launch)]
+            C.sizeof(_abi.ScatterBuf), C.sizeof(_abi.Launch), C.sizeof(_abi.BatchItem), C.sizeof(_abi.BwdOpts),
+            _abi.BwdOpts.graze.offset, _abi.NGRAD, _abi.GRAD_LNRHO, _abi.MAX_BINS]
     assert got == want
