@@ -14,16 +14,23 @@
 
 namespace mcd {
 
+// The record is written once, when the trajectory ends, as nine dword
+// stores marked non-temporal: records of one wavefront land in different
+// cache lines at different times, and streaming them past L2 keeps them from
+// evicting the loss tables and from being read back to merge partial lines.
 __device__ __forceinline__ void write_rec(mcd_electron_rec* rec, float E, const float* d, const float* p, uint32_t nsc,
                                           int o1, int o2, uint32_t flags) {
-  rec->final_E = E;
-  rec->dir[0] = d[0]; rec->dir[1] = d[1]; rec->dir[2] = d[2];
-  rec->pos[0] = p[0]; rec->pos[1] = p[1]; rec->pos[2] = p[2];
-  rec->n_scatt = nsc;
-  rec->outcome = (uint8_t)o1;
-  rec->outcome2 = (uint8_t)o2;
-  rec->flags = (uint8_t)flags;
-  rec->pad = 0;
+  uint32_t* w = reinterpret_cast<uint32_t*>(rec);
+  __builtin_nontemporal_store(__float_as_uint(E), w + 0);
+  __builtin_nontemporal_store(__float_as_uint(d[0]), w + 1);
+  __builtin_nontemporal_store(__float_as_uint(d[1]), w + 2);
+  __builtin_nontemporal_store(__float_as_uint(d[2]), w + 3);
+  __builtin_nontemporal_store(__float_as_uint(p[0]), w + 4);
+  __builtin_nontemporal_store(__float_as_uint(p[1]), w + 5);
+  __builtin_nontemporal_store(__float_as_uint(p[2]), w + 6);
+  __builtin_nontemporal_store(nsc, w + 7);
+  // outcome, outcome2, flags, pad (bytes 32..35, little endian)
+  __builtin_nontemporal_store((uint32_t)(o1 & 0xFF) | ((uint32_t)(o2 & 0xFF) << 8) | ((flags & 0xFFu) << 16), w + 8);
 }
 
 // wave-level append of scatter rows; returns the slot or ~0 when full
