@@ -7,7 +7,9 @@ Philox seed 1.  One step = one mcd_forward launch over the whole electron batch
 (tables built beforehand, outputs resident in HBM).  The channel-pick table and
 the W tables are built once outside the timed region.
 
-The second half of the metric, the backward/forward time ratio, is measured
+It also reports the 64-configuration batched launch against 64 sequential
+launches (same GPU, same Philox keys).  The second half of the metric, the
+backward/forward time ratio, is measured
 on the differentiation workload: Cu 20 keV -> 1 keV, s_tilt 70 deg, DS
 (Wc 50 eV) and continuous JL, 2^24 electrons, seed 4; forward = one
 mcd_forward launch, backward = one mcd_backward launch (vector-Jacobian
@@ -46,6 +48,7 @@ def parse():
     ap.add_argument("--waves-per-simd", type=int, default=0)
     ap.add_argument("--no-backward", action="store_true", help="skip the backward/forward ratio")
     ap.add_argument("--bwd-electrons", type=int, default=1 << 24)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-configuration batched comparison")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one device-to-device copy of this size (the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE readings)")
@@ -129,6 +132,48 @@ def backward_ratio(n, reps=3):
     return out
 
 
+def batched_ratio(reps=3):
+    """64 configurations: 64 sequential mcd_forward launches vs one autotuned,
+    graph-replayed batched launch (same Philox keys)."""
+    import torch
+
+    from mcd import _abi, autotune, engine, workloads
+
+    dev = torch.device("cuda", 0)
+    cfgs = workloads.batched_configs()
+    trs = workloads.build_transports(cfgs)
+    items = [(tr, c["n"]) for tr, c in zip(trs, cfgs)]
+    els = [torch.empty(c["n"] * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev) for c in cfgs]
+    cnt = torch.zeros(len(cfgs) * _abi.NCOUNTERS, dtype=torch.int64, device=dev)
+
+    def seq():
+        for i, (tr, c) in enumerate(zip(trs, cfgs)):
+            tr.launch(c["n"], els[i], cnt[i * _abi.NCOUNTERS:(i + 1) * _abi.NCOUNTERS], seed=3, config_id=i)
+
+    def best(f):
+        f()
+        torch.cuda.synchronize(dev)
+        t_best = float("inf")
+        for _ in range(reps):
+            torch.cuda.synchronize(dev)
+            t = time.perf_counter()
+            f()
+            torch.cuda.synchronize(dev)
+            t_best = min(t_best, time.perf_counter() - t)
+        return t_best * 1e3
+
+    t_seq = best(seq)
+    tuned = autotune.tune_batch(items, seed=3, reps=1, use_cache=False)
+    b = engine.Batch(items, pick_nodes=tuned["pick_nodes"])
+    t_bat = best(lambda: b.launch(seed=3, use_graph=True, block=tuned["block"], guard=tuned["guard"]))
+    return {"workload": "64 configurations {Al,Si,Cu,Au} x E0 {5,10,20,30} keV x tilt {0,70} deg x "
+                        "{DS, continuous JL}, Emin 0.5 keV, Wc 100 eV, tol 1e-3, "
+                        f"{sum(c['n'] for c in cfgs)} electrons (log-uniform 2^14..2^20, seed 3)",
+            "sequential_ms": t_seq, "batched_ms": t_bat, "ratio": t_seq / t_bat,
+            "tuned": {"block": tuned["block"], "pick_nodes": tuned["pick_nodes"], "guard": tuned["guard"],
+                      "candidates_ms": tuned["ms"]}}
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -209,6 +254,8 @@ def main():
         bw = backward_ratio(a.bwd_electrons)
         out["backward"] = bw
         out["bwd_fwd_ratio"] = bw["ratio_max"]
+    if not a.no_batched:
+        out["batched"] = batched_ratio()
     if not a.no_cpu_baseline:
         c_base, b1 = cpu_baselines(tr, a.cpu_seconds, a.seed)
         out["cpu_baseline"] = c_base

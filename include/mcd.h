@@ -184,8 +184,17 @@ typedef struct mcd_launch {
   int32_t waves_per_simd; /* occupancy variant: resident wavefronts per SIMD the kernel's
                              register budget guarantees (1 = compiler's choice, 4, 5, 6, 8) */
   int32_t precision;      /* MCD_PREC_FP32 transport (default) or MCD_PREC_FP64 (all-FP64 run) */
-  int32_t reserved;
+  int32_t flags;          /* MCD_LAUNCH_* */
 } mcd_launch;
+
+/* DS in FP32 can run the FP64 guard pick inline (one kernel) or deferred (two
+ * kernels: the first has no FP64 code and parks an electron whose channel pick
+ * falls in the guard band, the second resumes the parked electrons with the
+ * FP64 pick inline).  Records are identical either way.  Defaults: mcd_forward
+ * runs inline, mcd_batch_run deferred (the faster form of each on MI355X);
+ * these flags select one explicitly.  Setting both is an error. */
+#define MCD_LAUNCH_INLINE_GUARD 1
+#define MCD_LAUNCH_DEFER_GUARD 2
 
 typedef struct mcd_ctx mcd_ctx;
 typedef struct mcd_tables mcd_tables;

@@ -18,7 +18,9 @@ using namespace mcd;
 struct mcd_ctx {
   int device;
   int n_cu;
-  unsigned long long* d_queue;
+  unsigned long long* d_queue;  // [0] work counter, [1] parked electrons, [2] resume counter
+  ParkRec* d_park;              // deferred-guard park buffer
+  uint64_t park_cap;            // its capacity in electrons
   char err[512];
 };
 
@@ -90,6 +92,8 @@ extern "C" int mcd_ctx_create(int device, mcd_ctx** out) {
   mcd_ctx* c = new mcd_ctx();
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
+  c->d_park = nullptr;
+  c->park_cap = 0;
   c->err[0] = 0;
   if (hipMalloc(&c->d_queue, 64 * sizeof(unsigned long long)) != hipSuccess) {
     delete c;
@@ -103,6 +107,7 @@ extern "C" int mcd_ctx_destroy(mcd_ctx* ctx) {
   if (!ctx) return MCD_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipFree(ctx->d_queue);
+  if (ctx->d_park) (void)hipFree(ctx->d_park);
   delete ctx;
   return MCD_OK;
 }
@@ -513,13 +518,44 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   if (block % 64 || block > 256) return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64, at most 256", block);
   int minw = launch && launch->waves_per_simd > 0 ? launch->waves_per_simd : default_minw(*cfg);
   if (!valid_minw(minw)) return set_err(ctx, MCD_ERR_INVALID, "waves_per_simd %d not one of 1, 4, 5, 6, 8", minw);
-  int per_cu = max_blocks_per_cu(c, precision, minw, block);
-  uint64_t want = (n + block - 1) / block;
-  uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)per_cu;
-  if (grid > want) grid = want;
-  if (grid < 1) grid = 1;
-  HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
-  HIP_TRY(ctx, launch_transport(c, R, precision, minw, (int)grid, block, s));
+  const uint64_t want = (n + block - 1) / block;
+  const int flags = launch ? launch->flags : 0;
+  if ((flags & MCD_LAUNCH_INLINE_GUARD) && (flags & MCD_LAUNCH_DEFER_GUARD))
+    return set_err(ctx, MCD_ERR_INVALID, "MCD_LAUNCH_INLINE_GUARD and MCD_LAUNCH_DEFER_GUARD both set");
+  // the FP64 guard pick runs inline unless deferral to a second kernel is asked
+  // for (DS, FP32, no occupancy variant: those apply to the one-kernel form)
+  const bool deferred = cfg->mode == MCD_MODE_DS && precision == MCD_PREC_FP32 && minw == 1 &&
+                        (flags & MCD_LAUNCH_DEFER_GUARD);
+  if (deferred) {
+    if (ctx->park_cap < n) {
+      // the park buffer is reused across calls: wait for work that may still read it
+      HIP_TRY(ctx, hipDeviceSynchronize());
+      if (ctx->d_park) { (void)hipFree(ctx->d_park); ctx->d_park = nullptr; ctx->park_cap = 0; }
+      if (hipMalloc(&ctx->d_park, n * sizeof(ParkRec)) != hipSuccess) {
+        ctx->d_park = nullptr;
+        return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc of the park buffer (%llu electrons) failed",
+                       (unsigned long long)n);
+      }
+      ctx->park_cap = n;
+    }
+    R.park = ctx->d_park;
+    R.park_count = ctx->d_queue + 1;
+    R.resume_queue = ctx->d_queue + 2;
+    uint64_t g1 = (uint64_t)ctx->n_cu * (uint64_t)deferred_blocks_per_cu(c, 1, block);
+    uint64_t g2 = (uint64_t)ctx->n_cu * (uint64_t)deferred_blocks_per_cu(c, 2, block);
+    if (g1 > want) g1 = want;
+    if (g2 > want) g2 = want;
+    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 3, s));
+    HIP_TRY(ctx, launch_transport_deferred(c, R, 1, (int)(g1 < 1 ? 1 : g1), block, s));
+    HIP_TRY(ctx, launch_transport_deferred(c, R, 2, (int)(g2 < 1 ? 1 : g2), block, s));
+  } else {
+    int per_cu = max_blocks_per_cu(c, precision, minw, block);
+    uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)per_cu;
+    if (grid > want) grid = want;
+    if (grid < 1) grid = 1;
+    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
+    HIP_TRY(ctx, launch_transport(c, R, precision, minw, (int)grid, block, s));
+  }
   HIP_TRY(ctx, launch_count_outcomes(els, n, (unsigned long long*)counters, s));
   return MCD_OK;
 }
@@ -532,13 +568,15 @@ struct mcd_batch {
   int slice_floats;
   std::vector<mcd_tables*> own_tables;  // per-item pick tables sized for the LDS slices
   BatchItemDev* d_items;
-  unsigned long long* d_work;           // queues [n] then active [n]
+  unsigned long long* d_work;           // queues [n], active [n], parked [n], resumed [n]
+  ParkRec* d_park;                      // deferred guard: parked electrons, n_i per DS item
+  bool has_ds;
   hipStream_t capture_stream;
   hipGraphExec_t graph;
   uint64_t graph_seed;
   uint64_t* graph_counters;
   hipStream_t graph_stream;
-  int graph_block, graph_grid;
+  int graph_block, graph_grid, graph_deferred;
 };
 
 static double work_per_electron(const mcd_config& c) {
@@ -561,6 +599,7 @@ extern "C" int mcd_batch_destroy(mcd_batch* b) {
   for (mcd_tables* t : b->own_tables) mcd_tables_destroy(t);
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_work) (void)hipFree(b->d_work);
+  if (b->d_park) (void)hipFree(b->d_park);
   delete b;
   return MCD_OK;
 }
@@ -639,13 +678,30 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
   h[n_items - 1].cum_hi = 1.01f;
   b->slice_floats = batched_cfg_floats() + (slice > 0 ? slice : 4);
   if (hipMalloc(&b->d_items, n_items * sizeof(BatchItemDev)) != hipSuccess ||
-      hipMalloc(&b->d_work, 2 * (size_t)n_items * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&b->d_work, 4 * (size_t)n_items * sizeof(unsigned long long)) != hipSuccess) {
     mcd_batch_destroy(b);
     return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the batch failed");
   }
+  uint64_t n_park = 0;
+  for (int i = 0; i < n_items; ++i)
+    if (items[i].cfg->mode == MCD_MODE_DS) n_park += items[i].n;
+  b->has_ds = n_park > 0;
+  if (n_park && hipMalloc(&b->d_park, n_park * sizeof(ParkRec)) != hipSuccess) {
+    b->d_park = nullptr;
+    mcd_batch_destroy(b);
+    return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc of the batch park buffer (%llu electrons) failed",
+                   (unsigned long long)n_park);
+  }
+  uint64_t off = 0;
   for (int i = 0; i < n_items; ++i) {
     h[i].R.queue = b->d_work + i;
     h[i].active = b->d_work + n_items + i;
+    h[i].R.park_count = b->d_work + 2 * n_items + i;
+    h[i].R.resume_queue = b->d_work + 3 * n_items + i;
+    if (items[i].cfg->mode == MCD_MODE_DS) {
+      h[i].R.park = b->d_park + off;
+      off += items[i].n;
+    }
   }
   hipError_t e = hipMemcpy(b->d_items, h.data(), n_items * sizeof(BatchItemDev), hipMemcpyHostToDevice);
   if (e != hipSuccess) { mcd_batch_destroy(b); return set_err(ctx, MCD_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e)); }
@@ -653,13 +709,20 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
   return MCD_OK;
 }
 
-static hipError_t batch_enqueue(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, int grid, int block,
-                                hipStream_t s) {
-  hipError_t e = launch_fill_u64(b->d_work, 0ull, 2 * b->n_items, s);
-  if (e == hipSuccess)
-    e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, (unsigned long long*)counters, grid, block, s);
-  if (e == hipSuccess) e = launch_count_outcomes_batched(b->d_items, b->n_items, (unsigned long long*)counters, s);
-  (void)ctx;
+static hipError_t batch_enqueue(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, bool deferred,
+                                int block, hipStream_t s) {
+  hipError_t e = launch_fill_u64(b->d_work, 0ull, 4 * b->n_items, s);
+  unsigned long long* cnt = (unsigned long long*)counters;
+  if (deferred) {
+    const int g1 = ctx->n_cu * batched_blocks_per_cu(1, block, b->slice_floats);
+    const int g2 = ctx->n_cu * batched_blocks_per_cu(2, block, b->slice_floats);
+    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 1, g1, block, s);
+    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 2, g2, block, s);
+  } else {
+    const int g0 = ctx->n_cu * batched_blocks_per_cu(0, block, b->slice_floats);
+    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 0, g0, block, s);
+  }
+  if (e == hipSuccess) e = launch_count_outcomes_batched(b->d_items, b->n_items, cnt, s);
   return e;
 }
 
@@ -671,18 +734,22 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const int block = launch && launch->block > 0 ? launch->block : 256;
   if (block % 64 || block > 256) return set_err(ctx, MCD_ERR_INVALID, "block %d invalid", block);
-  const int grid = ctx->n_cu * batched_blocks_per_cu(block, b->slice_floats);
+  const int flags = launch ? launch->flags : 0;
+  if ((flags & MCD_LAUNCH_INLINE_GUARD) && (flags & MCD_LAUNCH_DEFER_GUARD))
+    return set_err(ctx, MCD_ERR_INVALID, "MCD_LAUNCH_INLINE_GUARD and MCD_LAUNCH_DEFER_GUARD both set");
+  // DS items defer the FP64 guard pick to a second phase unless asked not to
+  const bool deferred = b->has_ds && !(flags & MCD_LAUNCH_INLINE_GUARD);
   hipStream_t s = (hipStream_t)stream;
   if (!use_graph) {
-    HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, grid, block, s));
+    HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, deferred, block, s));
     return MCD_OK;
   }
   if (!b->graph || b->graph_seed != seed || b->graph_counters != counters || b->graph_block != block ||
-      b->graph_grid != grid) {
+      b->graph_deferred != (int)deferred) {
     if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
     if (!b->capture_stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
     HIP_TRY(ctx, hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = batch_enqueue(ctx, b, seed, counters, grid, block, b->capture_stream);
+    hipError_t e = batch_enqueue(ctx, b, seed, counters, deferred, block, b->capture_stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
     HIP_TRY(ctx, e);
@@ -693,7 +760,7 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
     b->graph_seed = seed;
     b->graph_counters = counters;
     b->graph_block = block;
-    b->graph_grid = grid;
+    b->graph_deferred = (int)deferred;
   }
   HIP_TRY(ctx, hipGraphLaunch(b->graph, s));
   return MCD_OK;

@@ -7,6 +7,16 @@
 
 namespace mcd {
 
+// An electron parked by the deferred-guard transport at the start of the step
+// whose channel pick needs FP64: the resume kernel restarts that step from
+// this state and the same Philox counter.
+struct ParkRec {
+  float Ehi, Elo;
+  float p[3], pc[3], d[3], y[3];
+  uint32_t step, flags;
+  uint64_t e;
+};
+
 struct RunArgs {
   uint64_t seed;
   uint64_t first, n;
@@ -16,6 +26,9 @@ struct RunArgs {
   unsigned long long* row_cursor;
   unsigned long long* counters;
   unsigned long long* queue;
+  ParkRec* park;                     // deferred guard: parked electrons (capacity n)
+  unsigned long long* park_count;    // electrons parked by the first kernel
+  unsigned long long* resume_queue;  // work counter of the resume kernel
 };
 
 // One configuration of a batched launch (device-resident).
@@ -28,8 +41,8 @@ struct BatchItemDev {
 };
 
 hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
-                          unsigned long long* counters, int grid, int block, hipStream_t s);
-int batched_blocks_per_cu(int block, int slice_floats);
+                          unsigned long long* counters, int phase, int grid, int block, hipStream_t s);
+int batched_blocks_per_cu(int phase, int block, int slice_floats);
 int batched_cfg_floats();
 hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
                                          hipStream_t s);
@@ -63,6 +76,9 @@ hipError_t launch_backward(const DevCfg& c, const BwdArgs& A, bool jac, int grid
 hipError_t launch_transport(const DevCfg& c, const RunArgs& R, int precision, int minw, int grid, int block,
                             hipStream_t s);
 int max_blocks_per_cu(const DevCfg& c, int precision, int minw, int block);
+// DS FP32 with the FP64 guard deferred: phase 1 parks guarded steps, phase 2 resumes them
+hipError_t launch_transport_deferred(const DevCfg& c, const RunArgs& R, int phase, int grid, int block, hipStream_t s);
+int deferred_blocks_per_cu(const DevCfg& c, int phase, int block);
 bool valid_minw(int w);
 hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsigned long long* counters, hipStream_t s);
 

@@ -63,11 +63,18 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const
 // configuration: refill idle lanes from the configuration's work counter,
 // step every active lane, write the record when a trajectory ends; returns
 // when the counter is exhausted and every lane is idle.
-template <int MODE, int NCH>
+// PHASE 0: the FP64 guard pick inline.  PHASE 1 (DS): no FP64 code; a step
+// whose pick falls in the guard band parks the electron (state at the start
+// of the step) and the lane takes the next electron.  PHASE 2: refill from the
+// parked electrons instead of new ones, guard inline.  Phases 1 + 2 give the
+// same records as phase 0: a parked step is redone from the same state and
+// Philox counter.
+template <int MODE, int NCH, int PHASE = 0>
 __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds, const RunArgs& R,
                                                uint32_t& n_events, uint32_t& n_fp64) {
   const int lane = threadIdx.x & 63;
   const uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
+  const uint64_t n_work = PHASE == 2 ? (uint64_t)__atomic_load_n(R.park_count, __ATOMIC_RELAXED) : R.n;
 
   bool active = false, open = true;
   uint64_t e = 0;
@@ -82,24 +89,34 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
         int nidle = __popcll(idle);
         int leader = __ffsll((unsigned long long)idle) - 1;
         unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(R.queue, (unsigned long long)nidle);
+        if (lane == leader) base = atomicAdd(PHASE == 2 ? R.resume_queue : R.queue, (unsigned long long)nidle);
         uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
         base = ((uint64_t)hi << 32) | lo;
         if (!active) {
           uint64_t idx = base + __popcll(idle & ((1ull << lane) - 1ull));
-          if (idx < R.n) {
+          if (idx < n_work) {
             active = true;
-            e = R.first + idx;
-            step = 0;
-            flags = 0;
-            E.hi = c.E0_hi; E.lo = c.E0_lo;
-            p[0] = p[1] = p[2] = 0.f;
-            pc[0] = pc[1] = pc[2] = 0.f;
-            d[0] = c.dir0_f[0]; d[1] = c.dir0_f[1]; d[2] = c.dir0_f[2];
-            y[0] = 0.f; y[1] = 1.f; y[2] = 0.f;
+            if constexpr (PHASE == 2) {
+              const ParkRec& q = R.park[idx];
+              e = q.e;
+              step = q.step;
+              flags = q.flags;
+              E.hi = q.Ehi; E.lo = q.Elo;
+#pragma unroll
+              for (int j = 0; j < 3; ++j) { p[j] = q.p[j]; pc[j] = q.pc[j]; d[j] = q.d[j]; y[j] = q.y[j]; }
+            } else {
+              e = R.first + idx;
+              step = 0;
+              flags = 0;
+              E.hi = c.E0_hi; E.lo = c.E0_lo;
+              p[0] = p[1] = p[2] = 0.f;
+              pc[0] = pc[1] = pc[2] = 0.f;
+              d[0] = c.dir0_f[0]; d[1] = c.dir0_f[1]; d[2] = c.dir0_f[2];
+              y[0] = 0.f; y[1] = 1.f; y[2] = 0.f;
+            }
           }
         }
-        if (base + (uint64_t)nidle >= R.n) open = false;
+        if (base + (uint64_t)nidle >= n_work) open = false;
       }
     }
     if (!__any(active)) break;
@@ -122,21 +139,41 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       float mfp;
       PickEval<NCH> pe;
       ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp, pe);
-      if (ch < 0) {  // guard band: settle the pick in FP64
-        // a crossing cell's swap was applied to pe.perm by the interpolation; undo it
-        // so d_pick_ordered sees node k's order and decides the pair exactly
-        uint32_t plo = pe.perm_lo, phi = pe.perm_hi;
-        if (pe.swap != (int)kMetaNoSwap && pe.pB > pe.pA) {
-          uint64_t pm = ((uint64_t)phi << 32) | plo;
-          const uint64_t la = (pm >> (4 * pe.swap)) & 15u, lb = (pm >> (4 * (pe.swap + 1))) & 15u;
-          pm &= ~((uint64_t)0xFFu << (4 * pe.swap));
-          pm |= (lb << (4 * pe.swap)) | (la << (4 * (pe.swap + 1)));
-          plo = (uint32_t)pm;
-          phi = (uint32_t)(pm >> 32);
+      if constexpr (PHASE == 1) {
+        if (ch < 0) {  // guard band: park the electron for the resume kernel
+          const uint64_t m = __ballot(true);
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          unsigned long long base = 0;
+          if (lane == leader) base = atomicAdd(R.park_count, (unsigned long long)__popcll(m));
+          const uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+          const uint64_t slot = (((uint64_t)hi << 32) | lo) + __popcll(m & ((1ull << lane) - 1ull));
+          ParkRec& q = R.park[slot];
+          q.Ehi = E.hi; q.Elo = E.lo;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) { q.p[j] = p[j]; q.pc[j] = pc[j]; q.d[j] = d[j]; q.y[j] = y[j]; }
+          q.step = step;
+          q.flags = flags;
+          q.e = e;
+          active = false;
+          continue;
         }
-        ch = d_pick_ordered<NCH>((double)E.hi + (double)E.lo, (double)u0, c, plo, phi, pe.swap, pe.exact);
-        sflags |= MCD_FLAG_FP64_PICK;
-        n_fp64++;
+      } else {
+        if (ch < 0) {  // guard band: settle the pick in FP64
+          // a crossing cell's swap was applied to pe.perm by the interpolation; undo it
+          // so d_pick_ordered sees node k's order and decides the pair exactly
+          uint32_t plo = pe.perm_lo, phi = pe.perm_hi;
+          if (pe.swap != (int)kMetaNoSwap && pe.pB > pe.pA) {
+            uint64_t pm = ((uint64_t)phi << 32) | plo;
+            const uint64_t la = (pm >> (4 * pe.swap)) & 15u, lb = (pm >> (4 * (pe.swap + 1))) & 15u;
+            pm &= ~((uint64_t)0xFFu << (4 * pe.swap));
+            pm |= (lb << (4 * pe.swap)) | (la << (4 * (pe.swap + 1)));
+            plo = (uint32_t)pm;
+            phi = (uint32_t)(pm >> 32);
+          }
+          ch = d_pick_ordered<NCH>((double)E.hi + (double)E.lo, (double)u0, c, plo, phi, pe.swap, pe.exact);
+          sflags |= MCD_FLAG_FP64_PICK;
+          n_fp64++;
+        }
       }
       l = -mfp * __logf(u1);
       if (ch == MCD_CH_RUTH) {  // Rutherford_halfPol, scattering.py:125-152
@@ -241,7 +278,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
 
 // MINW: minimum resident wavefronts per SIMD the register allocation must
 // allow (1 = compiler's choice); the autotuner picks it per configuration.
-template <int MODE, int NCH, int MINW>
+template <int MODE, int NCH, int MINW, int PHASE = 0>
 __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R) {
   extern __shared__ float lds[];
   if (MODE == MCD_MODE_DS) {
@@ -250,7 +287,7 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
     __syncthreads();
   }
   uint32_t n_events = 0, n_fp64 = 0;
-  transport_loop<MODE, NCH>(c, lds, R, n_events, n_fp64);
+  transport_loop<MODE, NCH, PHASE>(c, lds, R, n_events, n_fp64);
   const int lane = threadIdx.x & 63;
   const uint64_t cnt[2] = {n_events, n_fp64};
   wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
@@ -265,14 +302,28 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
 // lanes are idle, moves to the configuration with the most remaining work per
 // active wavefront.  Initial placement spreads wavefronts over configurations
 // in proportion to their estimated work.
+// Work of an item in a phase: new electrons (phases 0, 1) or parked ones (2).
+template <int PHASE>
+__device__ __forceinline__ void item_work(const BatchItemDev& it, unsigned long long& done, unsigned long long& total) {
+  if (PHASE == 2) {
+    done = __atomic_load_n(it.R.resume_queue, __ATOMIC_RELAXED);
+    total = __atomic_load_n(it.R.park_count, __ATOMIC_RELAXED);
+  } else {
+    done = __atomic_load_n(it.R.queue, __ATOMIC_RELAXED);
+    total = it.R.n;
+  }
+}
+
+template <int PHASE>
 __device__ __forceinline__ int next_item(const BatchItemDev* __restrict__ items, int n_items, int lane) {
   float best = 0.f;
   int bi = -1;
   for (int c = lane; c < n_items; c += 64) {
-    const unsigned long long q = __atomic_load_n(items[c].R.queue, __ATOMIC_RELAXED);
-    if (q < items[c].R.n) {
+    unsigned long long q, n;
+    item_work<PHASE>(items[c], q, n);
+    if (q < n) {
       const unsigned long long act = __atomic_load_n(items[c].active, __ATOMIC_RELAXED);
-      const float rem = (float)(items[c].R.n - q) * items[c].cost / (float)(act + 1);
+      const float rem = (float)(n - q) * items[c].cost / (float)(act + 1);
       if (rem > best) { best = rem; bi = c; }
     }
   }
@@ -288,14 +339,19 @@ __device__ __forceinline__ int next_item(const BatchItemDev* __restrict__ items,
 // followed by the channel-pick table, so the step loop reads both from LDS.
 constexpr int kCfgFloats = (int)((sizeof(DevCfg) + 15) / 16 * 4);
 
-template <int MODE, int NCH>
+int batched_cfg_floats() { return kCfgFloats; }
+
+template <int MODE, int NCH, int PHASE>
 __device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, const float* pick, uint64_t seed,
                                          uint32_t& ne, uint32_t& nf) {
   RunArgs R = itR;
   R.seed = seed;
-  transport_loop<MODE, NCH>(c, pick, R, ne, nf);
+  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0>(c, pick, R, ne, nf);
 }
 
+// PHASE 0: guard inline.  PHASE 1: DS items park guarded steps (no FP64 code
+// in the kernel).  PHASE 2: DS items resume their parked electrons.
+template <int PHASE>
 __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* __restrict__ items, int n_items,
                                                             int slice_floats, uint64_t seed,
                                                             unsigned long long* __restrict__ counters) {
@@ -304,14 +360,16 @@ __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* _
   const int wib = threadIdx.x >> 6;
   float* my = lds + (size_t)wib * slice_floats;
   float* pick = my + kCfgFloats;
-  const int waves_total = gridDim.x * (blockDim.x >> 6);
-  const int wave_id = blockIdx.x * (blockDim.x >> 6) + wib;
-  const float f = ((float)wave_id + 0.5f) / (float)waves_total;
   int ci = -1;
-  for (int c = 0; c < n_items; ++c)
-    if (f >= items[c].cum_lo && f < items[c].cum_hi) ci = c;
-  ci = __builtin_amdgcn_readfirstlane(ci);
-  if (ci < 0 || __atomic_load_n(items[ci].R.queue, __ATOMIC_RELAXED) >= items[ci].R.n) ci = next_item(items, n_items, lane);
+  if (PHASE != 2) {
+    const int waves_total = gridDim.x * (blockDim.x >> 6);
+    const int wave_id = blockIdx.x * (blockDim.x >> 6) + wib;
+    const float f = ((float)wave_id + 0.5f) / (float)waves_total;
+    for (int c = 0; c < n_items; ++c)
+      if (f >= items[c].cum_lo && f < items[c].cum_hi) ci = c;
+    ci = __builtin_amdgcn_readfirstlane(ci);
+  }
+  if (ci < 0 || __atomic_load_n(items[ci].R.queue, __ATOMIC_RELAXED) >= items[ci].R.n) ci = next_item<PHASE>(items, n_items, lane);
   while (ci >= 0) {
     const BatchItemDev& it = items[ci];
     if (lane == 0) atomicAdd(it.active, 1ull);
@@ -329,17 +387,17 @@ __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* _
     const DevCfg& c = *reinterpret_cast<const DevCfg*>(my);
     uint32_t ne = 0, nf = 0;
     if (mode == MCD_MODE_CONT) {
-      run_item<MCD_MODE_CONT, 4>(c, it.R, pick, seed, ne, nf);
+      if (PHASE != 2) run_item<MCD_MODE_CONT, 4, PHASE>(c, it.R, pick, seed, ne, nf);
     } else {
       switch (nch) {
-        case 4: run_item<MCD_MODE_DS, 4>(c, it.R, pick, seed, ne, nf); break;
-        case 5: run_item<MCD_MODE_DS, 5>(c, it.R, pick, seed, ne, nf); break;
-        case 6: run_item<MCD_MODE_DS, 6>(c, it.R, pick, seed, ne, nf); break;
-        case 7: run_item<MCD_MODE_DS, 7>(c, it.R, pick, seed, ne, nf); break;
-        case 8: run_item<MCD_MODE_DS, 8>(c, it.R, pick, seed, ne, nf); break;
-        case 9: run_item<MCD_MODE_DS, 9>(c, it.R, pick, seed, ne, nf); break;
-        case 10: run_item<MCD_MODE_DS, 10>(c, it.R, pick, seed, ne, nf); break;
-        default: run_item<MCD_MODE_DS, 11>(c, it.R, pick, seed, ne, nf); break;
+        case 4: run_item<MCD_MODE_DS, 4, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        case 5: run_item<MCD_MODE_DS, 5, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        case 6: run_item<MCD_MODE_DS, 6, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        case 7: run_item<MCD_MODE_DS, 7, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        case 8: run_item<MCD_MODE_DS, 8, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        case 9: run_item<MCD_MODE_DS, 9, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        case 10: run_item<MCD_MODE_DS, 10, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        default: run_item<MCD_MODE_DS, 11, PHASE>(c, it.R, pick, seed, ne, nf); break;
       }
     }
     const uint64_t cnt[2] = {ne, nf};
@@ -347,26 +405,36 @@ __global__ __launch_bounds__(256) void k_transport_batched(const BatchItemDev* _
     wave_add_counters(ic + MCD_CNT_EVENTS, cnt, 1, lane);
     wave_add_counters(ic + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
     if (lane == 0) atomicSub(it.active, 1ull);
-    ci = next_item(items, n_items, lane);
+    ci = next_item<PHASE>(items, n_items, lane);
     __builtin_amdgcn_wave_barrier();
   }
 }
 
-int batched_cfg_floats() { return kCfgFloats; }
+template <int PHASE>
+static const void* batched_kernel() {
+  return reinterpret_cast<const void*>(&k_transport_batched<PHASE>);
+}
 
 hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
-                          unsigned long long* counters, int grid, int block, hipStream_t s) {
+                          unsigned long long* counters, int phase, int grid, int block, hipStream_t s) {
   const size_t lds = (size_t)(block / 64) * slice_floats * sizeof(float);
-  hipLaunchKernelGGL(k_transport_batched, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
-                     counters);
+  if (phase == 1)
+    hipLaunchKernelGGL(k_transport_batched<1>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
+                       counters);
+  else if (phase == 2)
+    hipLaunchKernelGGL(k_transport_batched<2>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
+                       counters);
+  else
+    hipLaunchKernelGGL(k_transport_batched<0>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
+                       counters);
   return hipGetLastError();
 }
 
-int batched_blocks_per_cu(int block, int slice_floats) {
+int batched_blocks_per_cu(int phase, int block, int slice_floats) {
   int nb = 0;
   const size_t lds = (size_t)(block / 64) * slice_floats * sizeof(float);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_batched, block, lds) != hipSuccess || nb < 1)
-    nb = 1;
+  const void* k = phase == 1 ? batched_kernel<1>() : (phase == 2 ? batched_kernel<2>() : batched_kernel<0>());
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess || nb < 1) nb = 1;
   return nb;
 }
 
@@ -651,6 +719,53 @@ int max_blocks_per_cu(const DevCfg& c, int precision, int minw, int block) {
   }
   if (dispatch_f32(minw, c, nullptr, 0, block, 0, &nb) != hipSuccess || nb < 1) nb = 1;
   return nb;
+}
+
+// ---------------------------------------------------------------------------
+// Deferred guard (DS, FP32): phase 1 without FP64 code, phase 2 resumes.
+template <int NCH, int PHASE>
+static hipError_t deferred_nch(const DevCfg& c, const RunArgs* R, int grid, int block, size_t lds, hipStream_t s,
+                               int* occ) {
+  if (occ) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MCD_MODE_DS, NCH, 1, PHASE>, block, lds) !=
+        hipSuccess)
+      nb = 1;
+    *occ = nb < 1 ? 1 : nb;
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL((k_transport_f32<MCD_MODE_DS, NCH, 1, PHASE>), dim3(grid), dim3(block), lds, s, c, *R);
+  return hipGetLastError();
+}
+
+template <int PHASE>
+static hipError_t deferred_dispatch(const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s, int* occ) {
+  const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
+  switch (c.n_shells + 3) {
+    case 4: return deferred_nch<4, PHASE>(c, R, grid, block, lds, s, occ);
+    case 5: return deferred_nch<5, PHASE>(c, R, grid, block, lds, s, occ);
+    case 6: return deferred_nch<6, PHASE>(c, R, grid, block, lds, s, occ);
+    case 7: return deferred_nch<7, PHASE>(c, R, grid, block, lds, s, occ);
+    case 8: return deferred_nch<8, PHASE>(c, R, grid, block, lds, s, occ);
+    case 9: return deferred_nch<9, PHASE>(c, R, grid, block, lds, s, occ);
+    case 10: return deferred_nch<10, PHASE>(c, R, grid, block, lds, s, occ);
+    case 11: return deferred_nch<11, PHASE>(c, R, grid, block, lds, s, occ);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_transport_deferred(const DevCfg& c, const RunArgs& R, int phase, int grid, int block,
+                                     hipStream_t s) {
+  if (c.mode != MCD_MODE_DS) return hipErrorInvalidValue;
+  return phase == 1 ? deferred_dispatch<1>(c, &R, grid, block, s, nullptr)
+                    : deferred_dispatch<2>(c, &R, grid, block, s, nullptr);
+}
+
+int deferred_blocks_per_cu(const DevCfg& c, int phase, int block) {
+  int nb = 1;
+  hipError_t e = phase == 1 ? deferred_dispatch<1>(c, nullptr, 0, block, 0, &nb)
+                            : deferred_dispatch<2>(c, nullptr, 0, block, 0, &nb);
+  return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
 }  // namespace mcd

@@ -65,7 +65,11 @@ class BatchItem(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("waves_per_simd", C.c_int32), ("precision", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("flags", C.c_int32)]
+
+
+LAUNCH_INLINE_GUARD = 1
+LAUNCH_DEFER_GUARD = 2
 
 
 class BwdOpts(C.Structure):

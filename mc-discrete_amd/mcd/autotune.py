@@ -1,0 +1,115 @@
+"""Launch-shape autotuner for batched runs (mcd_batch_*).
+
+The batched kernel has three shape choices that trade against each other:
+
+* ``pick_nodes`` — the channel-pick table layout: nodes per configuration in
+  the wavefront's LDS slice.  More nodes give a narrower FP64 guard band and a
+  finer mean-free-path interpolation but a larger slice, so fewer workgroups
+  fit a CU; 0 sizes each table to the default 8 KB slice.
+* ``block`` — threads per workgroup (64, 128 or 256).
+* ``guard`` — where DS items take the FP64 guard pick: ``"deferred"`` (a
+  first phase without FP64 code parks those electrons, a second resumes them)
+  or ``"inline"`` (one phase, more registers, fewer resident wavefronts).
+
+Electrons are refilled into idle lanes every step (ballot + prefix count), so
+there is no separate compaction period to choose.
+
+``tune_batch`` times every candidate on the actual batch (graph replay, best
+of ``reps``), keeps the fastest and caches the choice by a key of the
+configurations and electron counts (JSON file under ``$MCD_TUNE_CACHE`` or
+``~/.cache/mcd``).  Different node counts interpolate the mean free path
+differently, so a run's records are reproducible for a given choice, not
+across choices.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+CANDIDATES = (
+    dict(block=256, pick_nodes=0, guard="deferred"),
+    dict(block=256, pick_nodes=128, guard="deferred"),
+    dict(block=256, pick_nodes=256, guard="deferred"),
+    dict(block=128, pick_nodes=0, guard="deferred"),
+    dict(block=128, pick_nodes=128, guard="deferred"),
+    dict(block=256, pick_nodes=0, guard="inline"),
+    dict(block=256, pick_nodes=128, guard="inline"),
+)
+
+
+def _name(c):
+    return f"block={c['block']},pick_nodes={c['pick_nodes']},guard={c.get('guard') or 'default'}"
+
+
+def _cache_path():
+    root = os.environ.get("MCD_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "mcd")
+    return os.path.join(root, "batch_tune.json")
+
+
+def batch_key(items) -> str:
+    """Key of a batch: every configuration field that shapes the work, and n."""
+    h = hashlib.sha1()
+    for tr, n in items:
+        c = tr.cfg
+        h.update(repr((c.mat.Z, round(c.mat.density, 9), c.E0, c.Emin, c.Wc, c.s_tilt, c.mode, c.bethe_model,
+                       c.thickness, c.max_scatt, round(c.J, 9), int(n))).encode())
+    return h.hexdigest()
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _time_candidate(items, cand, seed, reps, device):
+    import torch
+
+    from .engine import Batch
+
+    b = Batch(items, pick_nodes=cand["pick_nodes"], device=device)
+    b.launch(seed=seed, use_graph=True, block=cand["block"], guard=cand.get("guard"))
+    torch.cuda.synchronize(device)
+    best = float("inf")
+    for _ in range(reps):
+        b.counters.zero_()
+        torch.cuda.synchronize(device)
+        t = time.perf_counter()
+        b.launch(seed=seed, use_graph=True, block=cand["block"], guard=cand.get("guard"))
+        torch.cuda.synchronize(device)
+        best = min(best, time.perf_counter() - t)
+    del b
+    return best
+
+
+def tune_batch(items, seed=3, candidates=CANDIDATES, reps=2, device=0, use_cache=True, cache_path=None):
+    """Pick the fastest launch shape for `items` (list of (Transport, n)).
+
+    Returns {"block", "pick_nodes", "guard", "ms": {candidate: ms}, "cached": bool}."""
+    path = cache_path or _cache_path()
+    key = batch_key(items)
+    if use_cache:
+        hit = _load(path).get(key)
+        if hit and any(_name(hit) == _name(c) for c in candidates):
+            return dict(hit, cached=True)
+    timings = {}
+    for cand in candidates:
+        timings[_name(cand)] = 1e3 * _time_candidate(items, cand, seed, reps, device)
+    best = min(candidates, key=lambda c: timings[_name(c)])
+    out = dict(block=best["block"], pick_nodes=best["pick_nodes"], guard=best.get("guard"), ms=timings)
+    if use_cache:
+        data = _load(path)
+        data[key] = out
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + f".{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1)
+        os.replace(tmp, path)
+    return dict(out, cached=False)
+
+
+__all__ = ["CANDIDATES", "batch_key", "tune_batch"]

@@ -44,6 +44,17 @@ class RunResult:
         return int(self.counters[_abi.CNT_EVENTS])
 
 
+def guard_flags(guard):
+    """mcd_launch.flags for a guard placement: None, "inline" or "deferred"."""
+    if guard is None:
+        return 0
+    if guard == "inline":
+        return _abi.LAUNCH_INLINE_GUARD
+    if guard == "deferred":
+        return _abi.LAUNCH_DEFER_GUARD
+    raise ValueError(f"guard must be None, 'inline' or 'deferred', not {guard!r}")
+
+
 class Transport:
     """One configuration on one GPU."""
 
@@ -101,21 +112,24 @@ class Transport:
 
     # -- runs -----------------------------------------------------------------
     def launch(self, n, els, counters, seed=1, first=0, config_id=0, scat_rows=None, scat_max_electron=0,
-               precision=_abi.PREC_FP32, block=0, waves_per_simd=0, stream=None):
-        """Asynchronous launch on `stream` into caller-owned torch buffers."""
+               precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None, stream=None):
+        """Asynchronous launch on `stream` into caller-owned torch buffers.
+        guard: None (library default: inline), "inline" or "deferred" -- where
+        the DS FP64 guard pick runs (same records either way)."""
         ctx = context(self.device)
         sb = None
         if scat_rows is not None:
             sb = _abi.ScatterBuf(scat_rows.data_ptr(), scat_rows.numel() // _abi.SCATTER_DTYPE.itemsize,
                                  int(scat_max_electron))
-        la = _abi.Launch(int(block), int(waves_per_simd), int(precision), 1)
+        la = _abi.Launch(int(block), int(waves_per_simd), int(precision),
+                         guard_flags(guard))
         s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
         check(lib().mcd_forward(ctx, C.byref(self.cfg), self._tables, int(seed), int(config_id), int(first), int(n),
                                 els.data_ptr(), C.byref(sb) if sb is not None else None, counters.data_ptr(),
                                 C.byref(la), s), ctx)
 
     def run(self, n, seed=1, first=0, config_id=0, scats=False, scat_max_electron=None, scat_capacity=None,
-            precision=_abi.PREC_FP32, block=0, waves_per_simd=0) -> RunResult:
+            precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None) -> RunResult:
         """Run electrons [first, first+n) and return host results."""
         dev = torch.device("cuda", self.device)
         els = torch.empty(max(n, 1) * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
@@ -129,7 +143,7 @@ class Transport:
             rows = torch.empty(scat_capacity * _abi.SCATTER_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         self.launch(n, els, cnt, seed=seed, first=first, config_id=config_id, scat_rows=rows,
                     scat_max_electron=scat_max_electron or 0, precision=precision, block=block,
-                    waves_per_simd=waves_per_simd)
+                    waves_per_simd=waves_per_simd, guard=guard)
         torch.cuda.synchronize(dev)
         counters = cnt.cpu().numpy().view(np.uint64)
         e = els.cpu().numpy()[: n * _abi.ELECTRON_DTYPE.itemsize].view(_abi.ELECTRON_DTYPE)
@@ -225,9 +239,10 @@ class Batch:
         except Exception:
             pass
 
-    def launch(self, seed=3, use_graph=True, block=0, stream=None):
+    def launch(self, seed=3, use_graph=True, block=0, guard=None, stream=None):
+        """guard: None (library default: deferred), "inline" or "deferred"."""
         ctx = context(self.device)
-        la = _abi.Launch(int(block), 0, _abi.PREC_FP32, 0)
+        la = _abi.Launch(int(block), 0, _abi.PREC_FP32, guard_flags(guard))
         s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
         check(lib().mcd_batch_run(ctx, self._h, int(seed), self.counters.data_ptr(), C.byref(la), int(bool(use_graph)),
                                   s), ctx)

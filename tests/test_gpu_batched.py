@@ -25,17 +25,46 @@ def small_batch():
     return cfgs, trs
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batched_equals_per_configuration_launches(small_batch, use_graph):
+@pytest.mark.parametrize("use_graph,guard", [(False, None), (True, None), (True, "inline")])
+def test_batched_equals_per_configuration_launches(small_batch, use_graph, guard):
     from mcd import _abi, engine
     cfgs, trs = small_batch
     b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs, cfgs)], pick_nodes=128)
     for rep in range(2):  # graph replays must repeat the eager result
         b.counters.zero_()
-        b.launch(seed=3, use_graph=use_graph)
+        b.launch(seed=3, use_graph=use_graph, guard=guard)
         res = b.results()
         for i, (tr, c, r) in enumerate(zip(trs, cfgs, res)):
             ref = tr.run(c["n"], seed=3, config_id=i)
             assert r.els.tobytes() == ref.els.tobytes(), (i, c)
             assert int(r.counters[_abi.CNT_EVENTS]) == ref.events
             assert int(r.counters[_abi.CNT_ELECTRONS]) == c["n"]
+
+
+def test_autotuner_picks_and_caches_a_launch_shape(small_batch, tmp_path):
+    """The tuner times every candidate, returns the fastest, caches it by the
+    batch's configurations, and the tuned batch reproduces per-configuration
+    launches built with the same pick-table node count."""
+    import os
+    from mcd import autotune, engine, workloads
+    cfgs, trs = small_batch
+    items = [(tr, c["n"]) for tr, c in zip(trs, cfgs)]
+    cands = (dict(block=256, pick_nodes=128, guard="deferred"), dict(block=128, pick_nodes=128, guard="deferred"),
+             dict(block=256, pick_nodes=128, guard="inline"))
+    path = str(tmp_path / "tune.json")
+    first = autotune.tune_batch(items, candidates=cands, reps=1, cache_path=path)
+    assert not first["cached"]
+    assert (first["block"], first["pick_nodes"], first["guard"]) in {(256, 128, "deferred"), (128, 128, "deferred"),
+                                                                     (256, 128, "inline")}
+    assert set(first["ms"]) == {"block=256,pick_nodes=128,guard=deferred", "block=128,pick_nodes=128,guard=deferred",
+                                "block=256,pick_nodes=128,guard=inline"}
+    again = autotune.tune_batch(items, candidates=cands, reps=1, cache_path=path)
+    assert again["cached"] and again["block"] == first["block"] and again["guard"] == first["guard"]
+    other = workloads.batched_configs(scale=1 / 128)
+    assert autotune.batch_key(items) != autotune.batch_key([(tr, c["n"]) for tr, c in zip(trs, other)])
+    b = engine.Batch(items, pick_nodes=first["pick_nodes"])
+    b.launch(seed=3, use_graph=True, block=first["block"], guard=first["guard"])
+    res = b.results()
+    for i, (tr, c, r) in enumerate(zip(trs, cfgs, res)):
+        assert r.els.tobytes() == tr.run(c["n"], seed=3, config_id=i).els.tobytes(), i
+    assert os.path.exists(path)

@@ -254,3 +254,22 @@ def test_reference_statistical_checks_on_gpu_output(si10):
     u = np.random.default_rng(3).random(200)
     theory = np.degrees(np.arccos(1 - (2 * a * u) / (1 + a - u)))
     assert watson_u2(theory, mc) < 0.152
+
+
+def test_deferred_guard_equals_inline_guard(cu20, si10):
+    """The two-kernel DS form (FP64 guard picks parked and resumed in a second
+    kernel) writes the same records, scatter rows and counters as the
+    one-kernel form with the guard inline."""
+    tr = cu20[0]
+    n = 1 << 20
+    a = tr.run(n, seed=1, guard="deferred")
+    b = tr.run(n, seed=1)
+    assert a.els.tobytes() == b.els.tobytes()
+    for k in (_abi.CNT_EVENTS, _abi.CNT_BKS, _abi.CNT_ABS, _abi.CNT_FP64_PICKS):
+        assert int(a.counters[k]) == int(b.counters[k]), k
+    assert int(a.counters[_abi.CNT_FP64_PICKS]) > 0  # the resume kernel did run
+    trs = si10[0]
+    ra = trs.run(512, seed=0, scats=True, guard="deferred")
+    rb = trs.run(512, seed=0, scats=True, guard="inline")
+    assert ra.els.tobytes() == rb.els.tobytes()
+    assert ra.scats.tobytes() == rb.scats.tobytes()
