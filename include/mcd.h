@@ -181,8 +181,9 @@ enum {
 /* Launch tuning (0 = library default). */
 typedef struct mcd_launch {
   int32_t block;          /* threads per workgroup, multiple of 64, at most 256 */
-  int32_t waves_per_simd; /* occupancy variant: resident wavefronts per SIMD the kernel's
-                             register budget guarantees (1 = compiler's choice, 4, 5, 6, 8) */
+  int32_t waves_per_simd; /* occupancy variant of the one-kernel FP32 transport: resident
+                             wavefronts per SIMD its register budget guarantees (1 = compiler's
+                             choice, 4 = default, 5, 6, 8) */
   int32_t precision;      /* MCD_PREC_FP32 transport (default) or MCD_PREC_FP64 (all-FP64 run) */
   int32_t flags;          /* MCD_LAUNCH_* */
 } mcd_launch;

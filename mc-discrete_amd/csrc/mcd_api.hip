@@ -28,7 +28,7 @@ struct TableDev {
   float* Es = nullptr;
   float* Ws = nullptr;
   float* cdf = nullptr;
-  uint16_t* guide = nullptr;
+  uint8_t* rows = nullptr;
   int nE = 0, nW = 0;
   float E_first = 0.f, inv_h = 0.f;
 };
@@ -187,8 +187,9 @@ static DevCfg make_devcfg(const mcd_config& cfg, const mcd_tables* t, uint32_t c
     const TableDev* src[2] = {&t->m, &t->g};
     DevTable* dst[2] = {&c.tm, &c.tg};
     for (int q = 0; q < 2; ++q) {
-      dst[q]->Es = src[q]->Es; dst[q]->Ws = src[q]->Ws; dst[q]->cdf = src[q]->cdf; dst[q]->guide = src[q]->guide;
+      dst[q]->Es = src[q]->Es; dst[q]->Ws = src[q]->Ws; dst[q]->cdf = src[q]->cdf; dst[q]->rows = src[q]->rows;
       dst[q]->nE = src[q]->nE; dst[q]->nW = src[q]->nW;
+      dst[q]->narrow = src[q]->nW - 1 <= 255;
       dst[q]->E_first = src[q]->E_first; dst[q]->inv_h = src[q]->inv_h;
     }
   }
@@ -197,8 +198,8 @@ static DevCfg make_devcfg(const mcd_config& cfg, const mcd_tables* t, uint32_t c
     c.pick.n_nodes = t->n_nodes;
     c.pick.stride = t->stride;
     c.pick.n_ch = t->n_ch;
-    c.pick.lnE0 = (float)t->lnE0;
-    c.pick.inv_dlnE = (float)(1.0 / t->dln);
+    c.pick.log2E0 = (float)(t->lnE0 / M_LN2);
+    c.pick.inv_dlog2E = (float)(M_LN2 / t->dln);
     c.pick.delta = t->delta;
   }
   return c;
@@ -208,7 +209,7 @@ static void free_table(TableDev& t) {
   if (t.Es) (void)hipFree(t.Es);
   if (t.Ws) (void)hipFree(t.Ws);
   if (t.cdf) (void)hipFree(t.cdf);
-  if (t.guide) (void)hipFree(t.guide);
+  if (t.rows) (void)hipFree(t.rows);
   t = TableDev();
 }
 
@@ -241,7 +242,7 @@ static int upload_grid(mcd_ctx* ctx, const mcd_table_grid* gr, TableDev& td) {
   if (gr->nW - 1 > 65535) return set_err(ctx, MCD_ERR_TABLE, "tables wider than 65535 loss bins are not supported");
   if (hipMalloc(&td.Es, gr->nE * sizeof(float)) != hipSuccess || hipMalloc(&td.Ws, gr->nW * sizeof(float)) != hipSuccess ||
       hipMalloc(&td.cdf, ncdf * sizeof(float)) != hipSuccess ||
-      hipMalloc(&td.guide, (size_t)gr->nE * (kGuideQ + 1) * sizeof(uint16_t)) != hipSuccess)
+      hipMalloc(&td.rows, (size_t)gr->nE * kRowBytes) != hipSuccess)
     return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for a %d x %d table failed", gr->nE, gr->nW - 1);
   HIP_TRY(ctx, hipMemcpy(td.Es, gr->Es, gr->nE * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(td.Ws, gr->Ws, gr->nW * sizeof(float), hipMemcpyHostToDevice));
@@ -272,7 +273,7 @@ static void set_meta(float* node, int n_ch, uint32_t swap, bool exact) {
 static int build_pick_table(mcd_ctx* ctx, mcd_tables* t, int n_nodes) {
   const mcd_config& cfg = t->cfg;
   t->n_ch = cfg.mat.n_shells + 3;
-  t->stride = ((t->n_ch + 2) + 3) / 4 * 4;
+  t->stride = pick_stride(t->n_ch);
   t->n_nodes = n_nodes;
   if (t->nodes) { (void)hipFree(t->nodes); t->nodes = nullptr; }
   t->lnE0 = log(cfg.Emin);
@@ -389,10 +390,10 @@ extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
       }
     }
     for (int q = 0; q < 2; ++q) {
-      hipError_t e = launch_build_guide(td[q]->cdf, td[q]->nE, td[q]->nW, td[q]->guide, 0);
+      hipError_t e = launch_build_rows(td[q]->cdf, td[q]->Es, td[q]->nE, td[q]->nW, td[q]->rows, 0);
       if (e != hipSuccess) {
         mcd_tables_destroy(t);
-        return set_err(ctx, MCD_ERR_DEVICE, "guide build: %s", hipGetErrorString(e));
+        return set_err(ctx, MCD_ERR_DEVICE, "row-record build: %s", hipGetErrorString(e));
       }
     }
     if ((rc = build_pick_table(ctx, t, pick_nodes_default()))) {
@@ -431,7 +432,7 @@ extern "C" int mcd_tables_upload(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
     }
     hipError_t e = hipMemcpy(tmp, src[q], n * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = launch_mask(tmp, td[q]->cdf, n, 0);
-    if (e == hipSuccess) e = launch_build_guide(td[q]->cdf, td[q]->nE, td[q]->nW, td[q]->guide, 0);
+    if (e == hipSuccess) e = launch_build_rows(td[q]->cdf, td[q]->Es, td[q]->nE, td[q]->nW, td[q]->rows, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(tmp);
     if (e != hipSuccess) {
@@ -470,10 +471,14 @@ extern "C" int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap, i
 }
 
 // occupancy variant used when the caller does not choose one
-static int default_minw(const mcd_config& cfg) {
+// Occupancy of the one-kernel FP32 transport: 4 resident wavefronts per SIMD
+// (128 VGPRs, no spills) hides more of the table and LDS latency than the
+// compiler's own allocation (3 per SIMD) at every configuration measured;
+// 6 and 8 spill.  MCD_MINW overrides it.
+static int default_minw(const mcd_config&) {
   const char* s = getenv("MCD_MINW");
   if (s) return atoi(s);
-  return cfg.mode == MCD_MODE_CONT ? 1 : 1;
+  return 4;
 }
 
 static bool same_config(const mcd_config& a, const mcd_config& b) {
@@ -523,9 +528,8 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   if ((flags & MCD_LAUNCH_INLINE_GUARD) && (flags & MCD_LAUNCH_DEFER_GUARD))
     return set_err(ctx, MCD_ERR_INVALID, "MCD_LAUNCH_INLINE_GUARD and MCD_LAUNCH_DEFER_GUARD both set");
   // the FP64 guard pick runs inline unless deferral to a second kernel is asked
-  // for (DS, FP32, no occupancy variant: those apply to the one-kernel form)
-  const bool deferred = cfg->mode == MCD_MODE_DS && precision == MCD_PREC_FP32 && minw == 1 &&
-                        (flags & MCD_LAUNCH_DEFER_GUARD);
+  // for (DS, FP32; waves_per_simd applies to the one-kernel form only)
+  const bool deferred = cfg->mode == MCD_MODE_DS && precision == MCD_PREC_FP32 && (flags & MCD_LAUNCH_DEFER_GUARD);
   if (deferred) {
     if (ctx->park_cap < n) {
       // the park buffer is reused across calls: wait for work that may still read it
@@ -639,7 +643,7 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
       own->m = it.tables->m;
       own->g = it.tables->g;
       own->has_w = true;
-      const int stride = ((it.cfg->mat.n_shells + 3 + 2) + 3) / 4 * 4;
+      const int stride = pick_stride(it.cfg->mat.n_shells + 3);
       int nodes = pick_nodes > 0 ? pick_nodes : pick_budget / stride;
       if (nodes > 512) nodes = 512;
       rc = build_pick_table(ctx, own, nodes);
@@ -656,8 +660,8 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
       c.pick.n_nodes = own->n_nodes;
       c.pick.stride = own->stride;
       c.pick.n_ch = own->n_ch;
-      c.pick.lnE0 = (float)own->lnE0;
-      c.pick.inv_dlnE = (float)(1.0 / own->dln);
+      c.pick.log2E0 = (float)(own->lnE0 / M_LN2);
+      c.pick.inv_dlog2E = (float)(M_LN2 / own->dln);
       c.pick.delta = own->delta;
     }
     memset(&h[i], 0, sizeof(BatchItemDev));

@@ -140,7 +140,8 @@ hipError_t launch_bwd_nodes(const DevCfg& c, int n_nodes, double lnE0, double dl
   return hipGetLastError();
 }
 
-int bwd_stride(int mode, int n_ch) { return mode == MCD_MODE_CONT ? 4 : (4 + (n_ch - 1) + 3 * n_ch + 3) / 4 * 4; }
+// odd, so that the nodes read by the lanes of a wavefront start in different LDS banks
+int bwd_stride(int mode, int n_ch) { return (mode == MCD_MODE_CONT ? 4 : 4 + (n_ch - 1) + 3 * n_ch) | 1; }
 
 // ---------------------------------------------------------------------------
 // Replay kernel.
@@ -176,6 +177,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
   float accv = 0.f, accg[3] = {0.f, 0.f, 0.f};
 
   bool active = false, open = true;
+  WaveQueue wq;
   uint64_t e = 0;
   uint32_t step = 0;
   float E = 0.f, z = 0.f, w = 0.f;
@@ -187,14 +189,8 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     if (open) {
       const uint64_t idle = __ballot(!active);
       if (idle) {
-        const int nidle = __popcll(idle);
-        const int leader = __ffsll((unsigned long long)idle) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(A.queue, (unsigned long long)nidle);
-        const uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
-        base = ((uint64_t)hi << 32) | lo;
+        const uint64_t idx = refill_index(wq, A.queue, idle, lane, A.n, open);
         if (!active) {
-          const uint64_t idx = base + __popcll(idle & ((1ull << lane) - 1ull));
           if (idx < A.n) {
             active = true;
             e = A.first + idx;
@@ -215,7 +211,6 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
             dd[1][2] = kDeg * d[0];
           }
         }
-        if (base + (uint64_t)nidle >= A.n) open = false;
       }
     }
     if (!__any(active)) break;
@@ -229,7 +224,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     if (MODE == MCD_MODE_DS) dlnE[1] = 0.f;  // DS energies do not depend on the tilt
 
     // node interpolation
-    float tn = (__logf(E) - A.lnE0) * A.inv_dln;
+    float tn = (ln_f(E) - A.lnE0) * A.inv_dln;
     tn = fminf(fmaxf(tn, 0.f), (float)(A.n_nodes - 1) - 1e-3f);
     const int kn = (int)tn;
     const float fn = tn - (float)kn;
@@ -237,7 +232,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     const float* n1 = n0 + stride;
     auto lerp = [&](int o) { return n0[o] + fn * (n1[o] - n0[o]); };
     const float lnl = lerp(0), aE = lerp(1);
-    const float lam = __expf(lnl);
+    const float lam = exp_f(lnl);
     float dlnl[3];
     dlnl[0] = aE * dlnE[0] + lerp(2);
     dlnl[1] = aE * dlnE[1];
@@ -317,7 +312,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     const float rr = d[2] < -A.graze ? -__builtin_amdgcn_rcpf(d[2] * lam) : 0.f;
     if (d[2] < -A.graze) x = z * rr;
     if (x < kXCut) {
-      const float q = __expf(-x);
+      const float q = exp_f(-x);
       const float idz = __builtin_amdgcn_rcpf(d[2]);
       float dx[3], dC[3];
 #pragma unroll
@@ -328,7 +323,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       contribute(C, dC);
       // survival weight and truncated free path
       const float U = q + u_path * (1.f - q);
-      l = -lam * __logf(U);
+      l = -lam * ln_f(U);
       const float kq = lam * (1.f - u_path) * q * __builtin_amdgcn_rcpf(U);
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -337,7 +332,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       }
       w *= 1.f - q;
     } else {
-      l = -lam * __logf(u_path);
+      l = -lam * ln_f(u_path);
 #pragma unroll
       for (int i = 0; i < 3; ++i) dl[i] = l * dlnl[i];
       if (d[2] < 0.f && z + l * d[2] <= 0.f) {
@@ -382,8 +377,8 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       } else {
         const DevTable& t = (ch == 1) ? c.tm : c.tg;
         const float Ec = E;
-        const int row = table_row(t, E, [Ec](float v) { return v < Ec; });
-        const int col = table_col(t, row, u_ang);
+        int row;
+        const int col = table_lookup(t, E, [Ec](float v) { return v < Ec; }, u_ang, row);
         W = t.Ws[col];
         const size_t cell = (size_t)row * (size_t)(t.nW - 1) + col;
         // the column pick's probability depends on the energy (likelihood ratio)
@@ -412,18 +407,18 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       // Bethe stopping power at the pre-step energy and its derivatives
       float Sv, dSdE, dSdJ;
       if (c.bethe == MCD_BETHE_CLASSICAL) {
-        const float L = __logf(c.bethe_c1_f * E);
+        const float L = ln_f(c.bethe_c1_f * E);
         Sv = c.bethe_k1_f * iE * L;
         dSdE = -Sv * iE + c.bethe_k1_f * iE * iE;
         dSdJ = -c.bethe_k1_f * iE / Jf;
       } else if (c.bethe == MCD_BETHE_JL) {
-        const float L = __logf(c.bethe_c1_f * (E + kJ));
+        const float L = ln_f(c.bethe_c1_f * (E + kJ));
         const float ie2 = __builtin_amdgcn_rcpf(E + kJ);
         Sv = c.bethe_k1_f * iE * L;
         dSdE = -Sv * iE + c.bethe_k1_f * iE * ie2;
         dSdJ = c.bethe_k1_f * iE * ((float)c.bethe_kk * ie2 - 1.f / Jf);
       } else {
-        Sv = c.expl_k_f * iE * __logf(E * c.expl_inv_Estar_f);
+        Sv = c.expl_k_f * iE * ln_f(E * c.expl_inv_Estar_f);
         dSdE = -Sv * iE + c.expl_k_f * iE * iE;
         dSdJ = 0.f;
       }

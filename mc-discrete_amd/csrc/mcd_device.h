@@ -23,32 +23,48 @@ constexpr float kPiF = 3.14159265358979f;
 constexpr float kMasked = 2.0f;              // device sentinel for masked CDF entries (> any u)
 constexpr int64_t kMaxStepsOff = 1ll << 24;  // step cap when maxScatt is off
 
-// Pick-table node layout: [B_0 .. B_{n-2}, ln mfp, perm_lo, perm_hi|meta, pad..]
+// Pick-table node layout: [B_0 .. B_{n-2}, log2(mfp ln 2), perm_lo, perm_hi|meta, pad..]
 // perm: 4-bit channel labels in sorted (pick) order, positions 0..7 in perm_lo,
 // 8..10 in bits 0..11 of perm_hi.  meta (bits 12..): cell [k, k+1] swap position
 // (bits 12..15, 15 = no swap) and exact-only flag (bit 16).
+// Floats per pick-table node: n_ch - 1 cumulative probabilities, log2 of
+// (mfp ln 2) and two label words, padded to an odd count so that the nodes
+// read by the 64 lanes of a wavefront (one node each) start in different LDS
+// banks.
+__host__ __device__ constexpr int pick_stride(int n_ch) { return (n_ch + 2) | 1; }
+
 constexpr uint32_t kMetaNoSwap = 15u;
 constexpr uint32_t kMetaExact = 1u << 16;
 
-// Quantile guide of a CDF row: guide[row*(kGuideQ+1) + q] = first column whose
-// CDF >= q/kGuideQ, so the column of u lies in [guide[q], guide[q+1]] with
-// q = floor(u*kGuideQ); most lookups read one guide pair and at most a few
-// CDF entries instead of a full bisection.
-constexpr int kGuideQ = 128;
+// Row records of a loss table.  Record i (kRowBytes bytes) holds the grid
+// energies Es[i-1], Es[i] that bracket row i (record 0: Es[0] twice) and the
+// quantile guide of row i's CDF: entry q = bisect_left(cdf_row, q / G), the
+// first column whose CDF >= q / G, so the column of u lies in
+// [guide[q], guide[q+1]] with q = floor(u G).  Tables with at most 255 loss
+// bins keep a uint8 guide with G = 512, wider ones a uint16 guide with G = 256.
+// A lookup reads the guessed row's bracket and guide pair in one round trip
+// and, when the pair differs, up to four CDF entries in a second.
+constexpr int kGuideNarrow = 512;
+constexpr int kGuideWide = 256;
+constexpr int kRowHeader = 8;
+constexpr int kRowBytes = 528;
+static_assert(kRowHeader + kGuideNarrow + 1 <= kRowBytes && kRowHeader + 2 * (kGuideWide + 1) <= kRowBytes,
+              "row record too small");
 
 struct DevTable {
   const float* Es;   // nE energies (float32 grid)
   const float* Ws;   // nW losses
   const float* cdf;  // nE x (nW-1), masked = kMasked
-  const uint16_t* guide;  // nE x (kGuideQ+1)
+  const uint8_t* rows;  // nE row records of kRowBytes
   int nE, nW;
+  int narrow;           // 1: uint8 guide (nW - 1 <= 255), 0: uint16 guide
   float E_first, inv_h;  // linspace index guess
 };
 
 struct DevPick {
   const float* nodes;  // n_nodes x stride floats (global; staged into LDS)
   int n_nodes, stride, n_ch;
-  float lnE0, inv_dlnE;  // node k sits at ln E = lnE0 + k / inv_dlnE
+  float log2E0, inv_dlog2E;  // node k sits at log2 E = log2E0 + k / inv_dlog2E
   float delta;           // guard band on the cumulative probabilities
 };
 
@@ -94,8 +110,65 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
   return {c0, c1, c2, c3};
 }
 
-// u = (2*(x>>9)+1) * 2^-24, exact in FP32.
-__device__ __forceinline__ float u01f(uint32_t x) { return (float)(2u * (x >> 9) + 1u) * 5.9604644775390625e-08f; }
+// ---------------------------------------------------------------------------
+// Work distribution of the persistent kernels.  A wavefront refills its idle
+// lanes from a reservation [next, end) of indices it has taken from a shared
+// counter; when the reservation is short it takes a block of at least
+// kRefillBlock indices with one atomic.  Every index is handed out exactly once
+// (blocks from the counter are disjoint, a reservation is only consumed by its
+// wavefront), and the wavefront keeps refilling until its reservation passes n,
+// so no reserved index is left behind.  Taking one index per idle lane instead
+// makes thousands of resident wavefronts queue on the counter's cache line for
+// every refill; with blocks the counter sees one atomic per kRefillBlock
+// electrons.
+constexpr uint64_t kRefillBlock = 16;
+
+struct WaveQueue {
+  uint64_t next = 0, end = 0;
+};
+
+// Index for this lane if it is idle (caller checks idx < n); clears `open`
+// once the reservation has passed n.  idle = ballot of idle lanes, nonzero.
+__device__ __forceinline__ uint64_t refill_index(WaveQueue& q, unsigned long long* counter, uint64_t idle, int lane,
+                                                 uint64_t n, bool& open) {
+  const uint64_t nidle = (uint64_t)__popcll(idle);
+  const uint64_t pfx = (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
+  const uint64_t avail = q.end - q.next;
+  uint64_t idx;
+  if (nidle > avail) {
+    const uint64_t fetch = nidle - avail > kRefillBlock ? nidle - avail : kRefillBlock;
+    const int leader = __ffsll((unsigned long long)idle) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)fetch);
+    const uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+    const uint64_t blk = ((uint64_t)hi << 32) | lo;
+    idx = pfx < avail ? q.next + pfx : blk + (pfx - avail);
+    q.next = blk + (nidle - avail);
+    q.end = blk + fetch;
+  } else {
+    idx = q.next + pfx;
+    q.next += nidle;
+  }
+  if (q.next >= n) open = false;
+  return idx;
+}
+
+// u = (2*(x>>9)+1) * 2^-24, exact in FP32: 1 + (x>>9) 2^-23 built from the
+// bits, minus 1 - 2^-24 (the difference is representable, so exact).
+__device__ __forceinline__ float u01f(uint32_t x) {
+  return __uint_as_float(0x3f800000u | (x >> 9)) - 0.99999994039535522461f;
+}
+
+// ln and exp from the hardware base-2 instructions (v_log_f32, v_exp_f32).
+// Every argument on the transport path is a normal, finite float (energies,
+// probabilities in [2^-24, 1), bounded exponents), so the subnormal scaling
+// and range handling of the library functions are not needed.
+constexpr float kLn2F = 0.69314718055994530942f;
+constexpr float kLog2eF = 1.44269504088896340736f;
+__device__ __forceinline__ float log2_f(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float exp2_f(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float ln_f(float x) { return __builtin_amdgcn_logf(x) * kLn2F; }
+__device__ __forceinline__ float exp_f(float x) { return __builtin_amdgcn_exp2f(x * kLog2eF); }
 __device__ __forceinline__ double u01d(uint32_t x) { return (double)(2u * (x >> 9) + 1u) * 5.9604644775390625e-08; }
 
 // ---------------------------------------------------------------------------
@@ -223,20 +296,21 @@ struct PickEval {
   float pA, pB;          // swap cells: interpolated probabilities of the two channels
   int swap;              // swap position or kMetaNoSwap
   bool exact;            // exact-only cell
-  float mfp;
+  float mfp_ln2;         // mean free path times ln 2: the step is -mfp_ln2 log2(u)
 };
 
 template <int NCH>
 __device__ __forceinline__ void f_pick_eval(const float* __restrict__ nodes, const DevPick& P, float E,
                                             PickEval<NCH>& o) {
   constexpr int NB = NCH - 1;
-  float x = (__logf(E) - P.lnE0) * P.inv_dlnE;
+  constexpr int S = pick_stride(NCH);
+  float x = (log2_f(E) - P.log2E0) * P.inv_dlog2E;
   x = fminf(fmaxf(x, 0.f), (float)(P.n_nodes - 1));
   const int k = min((int)x, P.n_nodes - 2);
   const float f = x - (float)k;
-  const float* a = nodes + k * P.stride;
-  const float* b = a + P.stride;
-  o.mfp = __expf(a[NB] + f * (b[NB] - a[NB]));
+  const float* a = nodes + k * S;
+  const float* b = a + S;
+  o.mfp_ln2 = exp2_f(a[NB] + f * (b[NB] - a[NB]));
   o.perm_lo = __float_as_uint(a[NB + 1]);
   const uint32_t meta = __float_as_uint(a[NB + 2]);
   o.perm_hi = meta & 0xFFFu;
@@ -277,12 +351,12 @@ __device__ __forceinline__ int perm_label(uint32_t lo, uint32_t hi, int pos) {
 }
 
 // Returns the channel label, or -1 when the guard band (or an exact-only cell)
-// requires the FP64 pick.  mfp is always set.
+// requires the FP64 pick.  mfp_ln2 is always set.
 template <int NCH>
 __device__ __forceinline__ int f_pick_table(const float* __restrict__ nodes, const DevPick& P, float E, float u,
-                                            float& mfp, PickEval<NCH>& o) {
+                                            float& mfp_ln2, PickEval<NCH>& o) {
   f_pick_eval<NCH>(nodes, P, E, o);
-  mfp = o.mfp;
+  mfp_ln2 = o.mfp_ln2;
   if (o.exact) return -1;
   if (o.swap != (int)kMetaNoSwap && fabsf(o.pA - o.pB) <= P.delta) return -1;
   int cnt = 0;
@@ -355,29 +429,59 @@ __device__ __forceinline__ int d_pick_ordered(double E, double u, const DevCfg& 
 // ---------------------------------------------------------------------------
 // Inverse-CDF table lookup (pickTable, scattering.py:63-94).
 // Row = first grid energy >= E (bisect_left), column = first CDF entry >= u.
-template <class EnergyLess>
-__device__ __forceinline__ int table_row(const DevTable& t, float Eguess, EnergyLess es_lt_E) {
-  // linspace guess, then one pair of grid energies settles bisect_left exactly
-  int i = (int)ceilf((Eguess - t.E_first) * t.inv_h);
-  i = max(1, min(i, t.nE - 1));
-  const float lo = t.Es[i - 1], hi = t.Es[i];
-  if (es_lt_E(lo) && !es_lt_E(hi)) return i;
-  int a = 0, b = t.nE;
-  while (a < b) {
-    const int mid = (a + b) >> 1;
-    if (es_lt_E(t.Es[mid])) a = mid + 1; else b = mid;
+__device__ __forceinline__ void guide_pair(const DevTable& t, const uint8_t* rec, float u, int& g0, int& g1) {
+  if (t.narrow) {
+    const int q = min((int)(u * (float)kGuideNarrow), kGuideNarrow - 1);
+    g0 = rec[kRowHeader + q];
+    g1 = rec[kRowHeader + q + 1];
+  } else {
+    const int q = min((int)(u * (float)kGuideWide), kGuideWide - 1);
+    const uint16_t* g = reinterpret_cast<const uint16_t*>(rec + kRowHeader) + q;
+    g0 = g[0];
+    g1 = g[1];
   }
-  return min(a, t.nE - 1);
 }
 
-__device__ __forceinline__ int table_col(const DevTable& t, int row, float u) {
-  const float* r = t.cdf + (size_t)row * (size_t)(t.nW - 1);
-  const int q = min((int)(u * (float)kGuideQ), kGuideQ - 1);
-  const uint16_t* g = t.guide + (size_t)row * (kGuideQ + 1) + q;
-  int lo = g[0], hi = g[1];
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (r[mid] < u) lo = mid + 1; else hi = mid;
+// Loss-table lookup (scattering.py:603-608,625-630): row = bisect_left(Es, E)
+// with es_lt_E(v) deciding v < E, col = bisect_left(cdf_row, u).  The
+// linspace guess of the row is checked against its bracket; the guide pair of
+// the guessed row is loaded in the same round trip.
+template <class EnergyLess>
+__device__ __forceinline__ int table_lookup(const DevTable& t, float Eguess, EnergyLess es_lt_E, float u, int& row) {
+  int i = (int)ceilf((Eguess - t.E_first) * t.inv_h);
+  i = max(1, min(i, t.nE - 1));
+  const uint8_t* rec = t.rows + (size_t)i * kRowBytes;
+  const float2 br = *reinterpret_cast<const float2*>(rec);
+  int g0, g1;
+  guide_pair(t, rec, u, g0, g1);
+  if (!(es_lt_E(br.x) && !es_lt_E(br.y))) {
+    int a = 0, b = t.nE;
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (es_lt_E(t.Es[mid])) a = mid + 1; else b = mid;
+    }
+    i = min(a, t.nE - 1);
+    rec = t.rows + (size_t)i * kRowBytes;
+    guide_pair(t, rec, u, g0, g1);
+  }
+  row = i;
+  const float* r = t.cdf + (size_t)i * (size_t)(t.nW - 1);
+  int lo = g0;
+  const int span = g1 - g0;
+  if (span > 0) {
+    if (span <= 4) {
+      // the CDF row is nondecreasing: col = g0 + #{j in [g0, g1) : cdf[j] < u};
+      // four independent loads, clamped into [g0, g1)
+      const int l = span - 1;
+      const float a0 = r[lo], a1 = r[lo + min(1, l)], a2 = r[lo + min(2, l)], a3 = r[lo + min(3, l)];
+      lo += (int)(a0 < u) + (int)(1 < span && a1 < u) + (int)(2 < span && a2 < u) + (int)(3 < span && a3 < u);
+    } else {
+      int hi = g1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (r[mid] < u) lo = mid + 1; else hi = mid;
+      }
+    }
   }
   return lo;
 }

@@ -44,6 +44,8 @@ hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floa
                           unsigned long long* counters, int phase, int grid, int block, hipStream_t s);
 int batched_blocks_per_cu(int phase, int block, int slice_floats);
 int batched_cfg_floats();
+
+
 hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
                                          hipStream_t s);
 
@@ -88,7 +90,7 @@ hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, do
 hipError_t launch_ratio_score(const float* c0, const float* cp, const float* cm, int nE, int nW, double inv_2delta,
                               float* out, hipStream_t s);
 hipError_t launch_row_score(const float* cdf, const float* Es, int nE, int nW, float* out, hipStream_t s);
-hipError_t launch_build_guide(const float* cdf, int nE, int nW, uint16_t* guide, hipStream_t s);
+hipError_t launch_build_rows(const float* cdf, const float* Es, int nE, int nW, uint8_t* rows, hipStream_t s);
 hipError_t launch_pick_nodes(const DevCfg& c, int n_nodes, double lnE0, double dln, float* nodes, int stride,
                              hipStream_t s);
 hipError_t launch_pick_check(const DevCfg& c, int n_checks, float* err, int* bad, hipStream_t s);

@@ -153,25 +153,39 @@ hipError_t launch_row_score(const float* cdf, const float* Es, int nE, int nW, f
   return hipGetLastError();
 }
 
-// Quantile guide: guide[row][q] = bisect_left(cdf_row, q / kGuideQ) (masked
-// entries hold kMasked, above every q).
-__global__ void k_build_guide(const float* __restrict__ cdf, int nE, int nW, uint16_t* __restrict__ guide) {
+// Row records (layout in mcd_device.h): bracket energies and the quantile
+// guide q -> bisect_left(cdf_row, q / G) (masked entries hold kMasked, above
+// every q / G).  One workgroup per row.
+__global__ void k_build_rows(const float* __restrict__ cdf, const float* __restrict__ Es, int nE, int nW, int narrow,
+                             uint8_t* __restrict__ rows) {
   const int row = blockIdx.x;
   const int nb = nW - 1;
   const float* r = cdf + (size_t)row * nb;
-  for (int q = threadIdx.x; q <= kGuideQ; q += blockDim.x) {
-    const float v = (float)q / (float)kGuideQ;
+  uint8_t* rec = rows + (size_t)row * kRowBytes;
+  if (threadIdx.x == 0) {
+    float2 br;
+    br.x = Es[row > 0 ? row - 1 : 0];
+    br.y = Es[row];
+    *reinterpret_cast<float2*>(rec) = br;
+  }
+  const int G = narrow ? kGuideNarrow : kGuideWide;
+  const int used = kRowHeader + (narrow ? 1 : 2) * (G + 1);
+  for (int b = used + threadIdx.x; b < kRowBytes; b += blockDim.x) rec[b] = 0;
+  for (int q = threadIdx.x; q <= G; q += blockDim.x) {
+    const float v = (float)q / (float)G;
     int lo = 0, hi = nb;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
       if (r[mid] < v) lo = mid + 1; else hi = mid;
     }
-    guide[(size_t)row * (kGuideQ + 1) + q] = (uint16_t)lo;
+    if (narrow) rec[kRowHeader + q] = (uint8_t)lo;
+    else reinterpret_cast<uint16_t*>(rec + kRowHeader)[q] = (uint16_t)lo;
   }
 }
 
-hipError_t launch_build_guide(const float* cdf, int nE, int nW, uint16_t* guide, hipStream_t s) {
-  hipLaunchKernelGGL(k_build_guide, dim3(nE), dim3(64), 0, s, cdf, nE, nW, guide);
+hipError_t launch_build_rows(const float* cdf, const float* Es, int nE, int nW, uint8_t* rows, hipStream_t s) {
+  const int narrow = nW - 1 <= 255;
+  hipLaunchKernelGGL(k_build_rows, dim3(nE), dim3(128), 0, s, cdf, Es, nE, nW, narrow, rows);
   return hipGetLastError();
 }
 
@@ -205,7 +219,7 @@ __global__ void k_pick_nodes(DevCfg c, int n_nodes, double lnE0, double dln, flo
   }
   float* o = nodes + (size_t)k * stride;
   for (int i = 0; i < n - 1; ++i) o[i] = (float)probs[i];
-  o[n - 1] = (float)log(1. / (c.atnd * s) * 1e14);
+  o[n - 1] = (float)log2(1. / (c.atnd * s) * 1e14 * M_LN2);  // log2(mfp ln 2)
   uint32_t lo = 0, hi = 0;
   for (int i = 0; i < n; ++i) {
     uint32_t l = (uint32_t)lab[order[i]];

@@ -14,6 +14,11 @@
 
 namespace mcd {
 
+// LDS of the one-configuration kernels: the pick table (nf floats), then the
+// two loss-table descriptors at a 16-byte aligned offset.
+__host__ __device__ constexpr int tab_offset_floats(int nf) { return (nf + 3) & ~3; }
+static size_t single_lds_bytes(int nf) { return (size_t)tab_offset_floats(nf) * sizeof(float) + 2 * sizeof(DevTable); }
+
 // The record is written once, when the trajectory ends, as nine dword
 // stores marked non-temporal: records of one wavefront land in different
 // cache lines at different times, and streaming them past L2 keeps them from
@@ -70,30 +75,30 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const
 // same records as phase 0: a parked step is redone from the same state and
 // Philox counter.
 template <int MODE, int NCH, int PHASE = 0>
-__device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds, const RunArgs& R,
+__device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds,
+                                               const DevTable* __restrict__ tabs, const RunArgs& R,
                                                uint32_t& n_events, uint32_t& n_fp64) {
   const int lane = threadIdx.x & 63;
-  const uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
+  uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
   const uint64_t n_work = PHASE == 2 ? (uint64_t)__atomic_load_n(R.park_count, __ATOMIC_RELAXED) : R.n;
 
   bool active = false, open = true;
+  WaveQueue wq;
   uint64_t e = 0;
   uint32_t step = 0, flags = 0;
   F2 E = {0.f, 0.f};
   float p[3] = {0.f, 0.f, 0.f}, pc[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
 
   while (true) {
+    // the key words stay loop-carried SGPR values (set where all lanes are
+    // converged): otherwise the compiler hoists all twenty round keys out of
+    // the loop and spills them to VGPR lanes, paying a v_readlane per round
+    asm volatile("" : "+s"(k0), "+s"(k1));
     if (open) {
       uint64_t idle = __ballot(!active);
       if (idle) {
-        int nidle = __popcll(idle);
-        int leader = __ffsll((unsigned long long)idle) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(PHASE == 2 ? R.resume_queue : R.queue, (unsigned long long)nidle);
-        uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
-        base = ((uint64_t)hi << 32) | lo;
+        const uint64_t idx = refill_index(wq, PHASE == 2 ? R.resume_queue : R.queue, idle, lane, n_work, open);
         if (!active) {
-          uint64_t idx = base + __popcll(idle & ((1ull << lane) - 1ull));
           if (idx < n_work) {
             active = true;
             if constexpr (PHASE == 2) {
@@ -116,7 +121,6 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
             }
           }
         }
-        if (base + (uint64_t)nidle >= n_work) open = false;
       }
     }
     if (!__any(active)) break;
@@ -136,9 +140,9 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
     if (MODE == MCD_MODE_DS) {
       const float u0 = u01f(r.x), u1 = u01f(r.y), u2 = u01f(r.z);
       u_az = u01f(r.w);
-      float mfp;
+      float mfp_ln2;
       PickEval<NCH> pe;
-      ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp, pe);
+      ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp_ln2, pe);
       if constexpr (PHASE == 1) {
         if (ch < 0) {  // guard band: park the electron for the resume kernel
           const uint64_t m = __ballot(true);
@@ -175,7 +179,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
           n_fp64++;
         }
       }
-      l = -mfp * __logf(u1);
+      l = -mfp_ln2 * log2_f(u1);
       if (ch == MCD_CH_RUTH) {  // Rutherford_halfPol, scattering.py:125-152
         Wh = 0.f;
         const float a = c.c_alpha_f * __builtin_amdgcn_rcpf(E.hi);
@@ -187,10 +191,10 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
         Wh = c.Epl_hi; Wl = c.Epl_lo;
         c2 = 1.f; s2 = 0.f;
       } else {  // Moller table, or the outer-shell Gryzinski table for all shells (:602-640)
-        const DevTable& t = (ch == MCD_CH_MOLLER) ? c.tm : c.tg;
+        const DevTable& t = tabs[ch == MCD_CH_MOLLER ? 0 : 1];  // LDS: no per-lane kernel-argument loads
         const F2 Ec = E;
-        const int row = table_row(t, E.hi, [Ec](float v) { return f_lt_f2(v, Ec); });
-        const int col = table_col(t, row, u2);
+        int row;
+        const int col = table_lookup(t, E.hi, [Ec](float v) { return f_lt_f2(v, Ec); }, u2, row);
         Wh = t.Ws[col];
         if (!f_lt_f2(Wh, E)) sflags |= MCD_FLAG_LOSS_GE_E;
         // binaryCollModel (:156-179): W <= (E - E_F)/2
@@ -216,14 +220,14 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       const float rr = (Ek + 511.f) * __builtin_amdgcn_rcpf(Ek + 1024.f);
       const float sig = c.ruth_k_f * rr * rr * __builtin_amdgcn_rcpf(Ek * Ek * a * (1.f + a));
       const float mfp = c.mfp_k_f * __builtin_amdgcn_rcpf(sig);
-      l = -mfp * __logf(u0);
+      l = -mfp * ln_f(u0);
       if (!(l < 1e4f)) sflags |= MCD_FLAG_LONG_PATH;
       // Bethe stopping power at the pre-step energy (stoppingPowers.py:112-215)
       float S;
       const float iE = __builtin_amdgcn_rcpf(E.hi);
-      if (c.bethe == MCD_BETHE_CLASSICAL) S = c.bethe_k1_f * iE * __logf(c.bethe_c1_f * E.hi);
-      else if (c.bethe == MCD_BETHE_JL) S = c.bethe_k1_f * iE * __logf(c.bethe_c1_f * (E.hi + c.bethe_kJ_f));
-      else S = c.expl_k_f * iE * __logf(E.hi * c.expl_inv_Estar_f);
+      if (c.bethe == MCD_BETHE_CLASSICAL) S = c.bethe_k1_f * iE * ln_f(c.bethe_c1_f * E.hi);
+      else if (c.bethe == MCD_BETHE_JL) S = c.bethe_k1_f * iE * ln_f(c.bethe_c1_f * (E.hi + c.bethe_kJ_f));
+      else S = c.expl_k_f * iE * ln_f(E.hi * c.expl_inv_Estar_f);
       Wh = l * S;
       if (!f_lt_f2(Wh, E)) sflags |= MCD_FLAG_LOSS_GE_E;
       const float om = 1.f - u1;
@@ -277,17 +281,25 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
 }
 
 // MINW: minimum resident wavefronts per SIMD the register allocation must
-// allow (1 = compiler's choice); the autotuner picks it per configuration.
+// allow (1 = compiler's choice, the default; others per launch).
 template <int MODE, int NCH, int MINW, int PHASE = 0>
 __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R) {
   extern __shared__ float lds[];
+  DevTable* tabs = nullptr;
   if (MODE == MCD_MODE_DS) {
+    // pick table, then the two loss-table descriptors (Moller, Gryzinski) that
+    // a lane selects by its channel
     const int nf = c.pick.n_nodes * c.pick.stride;
     for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
+    tabs = reinterpret_cast<DevTable*>(lds + tab_offset_floats(nf));
+    if (threadIdx.x == 0) {
+      tabs[0] = c.tm;
+      tabs[1] = c.tg;
+    }
     __syncthreads();
   }
   uint32_t n_events = 0, n_fp64 = 0;
-  transport_loop<MODE, NCH, PHASE>(c, lds, R, n_events, n_fp64);
+  transport_loop<MODE, NCH, PHASE>(c, lds, tabs, R, n_events, n_fp64);
   const int lane = threadIdx.x & 63;
   const uint64_t cnt[2] = {n_events, n_fp64};
   wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
@@ -296,8 +308,8 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
 
 // ---------------------------------------------------------------------------
 // Batched launch: many configurations in one persistent grid.  A wavefront
-// works on one configuration at a time (its channel-pick table staged in the
-// wavefront's own LDS slice, its constants read with uniform loads), refills
+// works on one configuration at a time (its constants and channel-pick table
+// staged in the wavefront's own LDS slice), refills
 // from that configuration's counter and, once the counter is exhausted and its
 // lanes are idle, moves to the configuration with the most remaining work per
 // active wavefront.  Initial placement spreads wavefronts over configurations
@@ -346,7 +358,8 @@ __device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, co
                                          uint32_t& ne, uint32_t& nf) {
   RunArgs R = itR;
   R.seed = seed;
-  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0>(c, pick, R, ne, nf);
+  // c is the wavefront's LDS copy, so its two table descriptors are in LDS too
+  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0>(c, pick, &c.tm, R, ne, nf);
 }
 
 // PHASE 0: guard inline.  PHASE 1: DS items park guarded steps (no FP64 code
@@ -491,8 +504,8 @@ __device__ __forceinline__ DStep d_step_physics(const DevCfg& c, double E, const
     } else {                                // :602-640
       const DevTable& t = (o.ch == MCD_CH_MOLLER) ? c.tm : c.tg;
       const double Ec = E;
-      const int row = table_row(t, (float)E, [Ec](float v) { return (double)v < Ec; });
-      const int col = table_col(t, row, (float)u2);
+      int row;
+      const int col = table_lookup(t, (float)E, [Ec](float v) { return (double)v < Ec; }, (float)u2, row);
       o.W = (double)t.Ws[col];
       if (!(o.W < E)) o.sflags |= MCD_FLAG_LOSS_GE_E;
       if (o.W <= (E - c.Ef) * 0.5) o.c2 = 0.5 * (sqrt(1 - (o.W / E)) + 1);
@@ -534,6 +547,7 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
   const int lane = threadIdx.x & 63;
   const uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
   bool active = false, open = true;
+  WaveQueue wq;
   uint64_t e = 0;
   uint32_t step = 0, flags = 0;
   double E = 0., p[3] = {0., 0., 0.}, d[3] = {0., 0., 0.}, y[3] = {0., 0., 0.};
@@ -542,14 +556,8 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
     if (open) {
       uint64_t idle = __ballot(!active);
       if (idle) {
-        int nidle = __popcll(idle);
-        int leader = __ffsll((unsigned long long)idle) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(R.queue, (unsigned long long)nidle);
-        uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
-        base = ((uint64_t)hi << 32) | lo;
+        const uint64_t idx = refill_index(wq, R.queue, idle, lane, R.n, open);
         if (!active) {
-          uint64_t idx = base + __popcll(idle & ((1ull << lane) - 1ull));
           if (idx < R.n) {
             active = true;
             e = R.first + idx;
@@ -560,7 +568,6 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
             y[0] = 0.; y[1] = 1.; y[2] = 0.;
           }
         }
-        if (base + (uint64_t)nidle >= R.n) open = false;
       }
     }
     if (!__any(active)) break;
@@ -679,7 +686,7 @@ static hipError_t dispatch_w(int minw, const DevCfg& c, const RunArgs* R, int gr
 static hipError_t dispatch_f32(int minw, const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s,
                                int* occ) {
   if (c.mode == MCD_MODE_CONT) return dispatch_w<MCD_MODE_CONT, 4>(minw, c, R, grid, block, 0, s, occ);
-  const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
+  const size_t lds = single_lds_bytes(c.pick.n_nodes * c.pick.stride);
   switch (c.n_shells + 3) {
     case 4: return dispatch_w<MCD_MODE_DS, 4>(minw, c, R, grid, block, lds, s, occ);
     case 5: return dispatch_w<MCD_MODE_DS, 5>(minw, c, R, grid, block, lds, s, occ);
@@ -740,7 +747,7 @@ static hipError_t deferred_nch(const DevCfg& c, const RunArgs* R, int grid, int 
 
 template <int PHASE>
 static hipError_t deferred_dispatch(const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s, int* occ) {
-  const size_t lds = (size_t)c.pick.n_nodes * c.pick.stride * sizeof(float);
+  const size_t lds = single_lds_bytes(c.pick.n_nodes * c.pick.stride);
   switch (c.n_shells + 3) {
     case 4: return deferred_nch<4, PHASE>(c, R, grid, block, lds, s, occ);
     case 5: return deferred_nch<5, PHASE>(c, R, grid, block, lds, s, occ);
