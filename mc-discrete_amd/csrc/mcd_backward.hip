@@ -443,8 +443,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       const float st = 2.f * __builtin_amdgcn_sqrtf(c2 * s2);
       const float kst = st > 0.f ? 2.f * ct * __builtin_amdgcn_rcpf(st) : 0.f;
       float sp, cp;
-      sincospif(2.f * u_az, &sp, &cp);
-      sp = fabsf(sp);
+      cos_abs_sin_2pi(u_az, cp, sp);
       const float w0 = y[1] * d[2] - y[2] * d[1];
       const float w1 = y[2] * d[0] - y[0] * d[2];
       const float w2 = y[0] * d[1] - y[1] * d[0];

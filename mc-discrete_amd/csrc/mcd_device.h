@@ -100,8 +100,9 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
     if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
     uint64_t p0 = (uint64_t)0xD2511F53u * c0;
     uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    // three-input XOR in one v_bitop3_b32 (truth table 0x96)
+    uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+    uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
     c1 = (uint32_t)p1;
     c3 = (uint32_t)p0;
     c0 = n0;
@@ -154,9 +155,10 @@ __device__ __forceinline__ uint64_t refill_index(WaveQueue& q, unsigned long lon
 }
 
 // u = (2*(x>>9)+1) * 2^-24, exact in FP32: 1 + (x>>9) 2^-23 built from the
-// bits, minus 1 - 2^-24 (the difference is representable, so exact).
+// bits ({0x7F, x} >> 9 = 0x3F800000 | x >> 9, one v_alignbit_b32), minus
+// 1 - 2^-24 (the difference is representable, so exact).
 __device__ __forceinline__ float u01f(uint32_t x) {
-  return __uint_as_float(0x3f800000u | (x >> 9)) - 0.99999994039535522461f;
+  return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, x, 9u)) - 0.99999994039535522461f;
 }
 
 // ln and exp from the hardware base-2 instructions (v_log_f32, v_exp_f32).
@@ -486,6 +488,29 @@ __device__ __forceinline__ int table_lookup(const DevTable& t, float Eguess, Ene
   return lo;
 }
 
+// cos(2 pi u) and |sin(2 pi u)| for u in (0, 1): the folded azimuth of the
+// direction update.  The reductions a = |u - rint(u)| in [0, 1/2],
+// q = rint(4a) in {0, 1, 2} and z = a - q/4 in [-1/8, 1/8] are exact; then
+// sin(2 pi z) = z P(z^2) and cos(2 pi z) = 1 + z^2 Q(z^2) with coefficients
+// fitted by weighted least squares on [0, 1/8] (within 1.7 ulp of the true
+// values at every u = (2m+1) 2^-24), and the quadrant q rotates the pair.
+__device__ __forceinline__ void cos_abs_sin_2pi(float u, float& cp, float& sp) {
+  const float a = fabsf(u - __builtin_rintf(u));
+  const float q = __builtin_rintf(4.f * a);
+  const float z = __builtin_fmaf(-0.25f, q, a);
+  const float w = z * z;
+  float ps = __builtin_fmaf(w, -75.43057250976562f, 81.59331512451172f);
+  ps = __builtin_fmaf(w, ps, -41.34166717529297f);
+  ps = __builtin_fmaf(w, ps, 6.2831854820251465f);
+  const float sn = z * ps;
+  float pc = __builtin_fmaf(w, 59.25483322143555f, -85.44401550292969f);
+  pc = __builtin_fmaf(w, pc, 64.9393310546875f);
+  pc = __builtin_fmaf(w, pc, -19.739208221435547f);
+  const float cs = __builtin_fmaf(w, pc, 1.f);
+  cp = q == 0.f ? cs : (q == 1.f ? -sn : -cs);
+  sp = q == 0.f ? sn : (q == 1.f ? cs : -sn);
+}
+
 // ---------------------------------------------------------------------------
 // Direction update, closed form of q_az * q_polar acting on (d, y_local)
 // (electron.py:87-99, rotation.py:41-53): with w = y x d,
@@ -496,8 +521,7 @@ __device__ __forceinline__ void f_rotate(float c2, float s2, float u_az, float* 
   float ct = c2 - s2;
   float st = 2.f * __builtin_amdgcn_sqrtf(c2 * s2);
   float sp, cp;
-  sincospif(2.f * u_az, &sp, &cp);
-  sp = fabsf(sp);
+  cos_abs_sin_2pi(u_az, cp, sp);
   float w0 = y[1] * d[2] - y[2] * d[1];
   float w1 = y[2] * d[0] - y[0] * d[2];
   float w2 = y[0] * d[1] - y[1] * d[0];
