@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=0)
+    ap.add_argument("--guard", choices=("inline", "deferred"), default=None,
+                    help="fix the FP64 guard placement (with --block/--waves-per-simd: skips the autotuner)")
     ap.add_argument("--no-backward", action="store_true", help="skip the backward/forward ratio")
     ap.add_argument("--bwd-electrons", type=int, default=1 << 24)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-configuration batched comparison")
@@ -189,8 +191,13 @@ def main():
     n = a.electrons
     els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    # launch shape (block, waves/SIMD, guard placement) from the autotuner,
+    # outside the timed region; an explicit --block/--waves-per-simd skips it
+    tuned = None
+    if not (a.block or a.waves_per_simd or a.guard):
+        tuned = tr.tune(n, seed=a.seed, use_cache=False)
     for _ in range(a.warmup):
-        tr.launch(n, els, cnt, seed=a.seed, block=a.block, waves_per_simd=a.waves_per_simd)
+        tr.launch(n, els, cnt, seed=a.seed, block=a.block, waves_per_simd=a.waves_per_simd, guard=a.guard)
     torch.cuda.synchronize(dev)
     cnt.zero_()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
@@ -199,7 +206,7 @@ def main():
     t0 = time.perf_counter()
     for k in range(a.steps):
         starts[k].record()
-        tr.launch(n, els, cnt, seed=a.seed, block=a.block, waves_per_simd=a.waves_per_simd)
+        tr.launch(n, els, cnt, seed=a.seed, block=a.block, waves_per_simd=a.waves_per_simd, guard=a.guard)
         ends[k].record()
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
@@ -249,6 +256,9 @@ def main():
                      "pipe": "VALU FP32", "ops_per_event": F_OPS_CU_DS,
                      "frac_vs_nonpacked_78_6": achieved / 78.6},
         "cpu_baseline": None,
+        "launch_shape": ({"block": tuned["block"], "waves_per_simd": tuned["waves_per_simd"], "guard": tuned["guard"],
+                          "autotuned": True, "candidates_ms": tuned["ms"]} if tuned is not None else
+                         {"block": a.block, "waves_per_simd": a.waves_per_simd, "guard": a.guard, "autotuned": False}),
     }
     if not a.no_backward:
         bw = backward_ratio(a.bwd_electrons)

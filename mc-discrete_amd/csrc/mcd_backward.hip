@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;
-  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
   const int stride = A.stride, n_ch = A.n_ch;
   const float Emin = c.Emin_hi;
   const float Epl = c.Epl_hi, Ef = c.Ef2_hi;
@@ -186,6 +186,9 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
   float dd[3][3], dy[3][3];  // [slot][component]; DS keeps slot 1 implicit (rigid tilt)
 
   while (true) {
+    // key words as loop-carried SGPR values, set where all lanes are converged
+    // (keeps the twenty round keys from being hoisted and spilled)
+    asm volatile("" : "+s"(k0), "+s"(k1));
     if (open) {
       const uint64_t idle = __ballot(!active);
       if (idle) {

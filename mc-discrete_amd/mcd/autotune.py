@@ -1,4 +1,13 @@
-"""Launch-shape autotuner for batched runs (mcd_batch_*).
+"""Launch-shape autotuners for single-configuration launches (mcd_forward) and
+batched runs (mcd_batch_*).
+
+Single configurations (``tune_transport``): ``block`` (threads per workgroup),
+``waves_per_simd`` (the register budget of the one-kernel transport: 1 leaves
+the allocation to the compiler, 4 and 5 cap it for more resident wavefronts)
+and ``guard`` (the FP64 guard pick inline, or deferred to a second kernel whose
+first phase carries no FP64 code).  Records are identical for every shape.
+
+Batched runs (``tune_batch``):
 
 The batched kernel has three shape choices that trade against each other:
 
@@ -43,9 +52,19 @@ def _name(c):
     return f"block={c['block']},pick_nodes={c['pick_nodes']},guard={c.get('guard') or 'default'}"
 
 
-def _cache_path():
+TRANSPORT_CANDIDATES = (
+    dict(block=256, waves_per_simd=4, guard="inline"),
+    dict(block=128, waves_per_simd=4, guard="inline"),
+    dict(block=256, waves_per_simd=1, guard="inline"),
+    dict(block=256, waves_per_simd=5, guard="inline"),
+    dict(block=256, waves_per_simd=0, guard="deferred"),
+    dict(block=128, waves_per_simd=0, guard="deferred"),
+)
+
+
+def _cache_path(name="batch_tune.json"):
     root = os.environ.get("MCD_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "mcd")
-    return os.path.join(root, "batch_tune.json")
+    return os.path.join(root, name)
 
 
 def batch_key(items) -> str:
@@ -56,6 +75,72 @@ def batch_key(items) -> str:
         h.update(repr((c.mat.Z, round(c.mat.density, 9), c.E0, c.Emin, c.Wc, c.s_tilt, c.mode, c.bethe_model,
                        c.thickness, c.max_scatt, round(c.J, 9), int(n))).encode())
     return h.hexdigest()
+
+
+def transport_key(tr, n) -> str:
+    """Key of one configuration's launch: the fields that shape its work, and n."""
+    return "t" + batch_key([(tr, n)])
+
+
+def _tname(c):
+    return f"block={c['block']},waves_per_simd={c['waves_per_simd']},guard={c['guard']}"
+
+
+def _time_transport(tr, n, cand, seed, reps, device):
+    import torch
+
+    from . import _abi
+    dev = torch.device("cuda", device)
+    els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    kw = dict(block=cand["block"], waves_per_simd=cand["waves_per_simd"], guard=cand["guard"])
+    saved, tr.tuned = tr.tuned, None
+    try:
+        tr.launch(n, els, cnt, seed=seed, **kw)
+        torch.cuda.synchronize(dev)
+        best = float("inf")
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            tr.launch(n, els, cnt, seed=seed, **kw)
+            b.record()
+            torch.cuda.synchronize(dev)
+            best = min(best, a.elapsed_time(b))
+    finally:
+        tr.tuned = saved
+    return best
+
+
+def _store(path, key, value):
+    data = _load(path)
+    data[key] = value
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + f".{os.getpid()}.tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1)
+    os.replace(tmp, path)
+
+
+def tune_transport(tr, n=1 << 22, seed=1, candidates=TRANSPORT_CANDIDATES, reps=2, device=0, use_cache=True,
+                   cache_path=None):
+    """Pick the fastest launch shape for n electrons of Transport `tr` (FP32).
+
+    Returns {"block", "waves_per_simd", "guard", "ms": {candidate: ms}, "cached": bool}."""
+    path = cache_path or _cache_path("transport_tune.json")
+    key = transport_key(tr, n)
+    from . import _abi
+    if tr.mode != _abi.MODE_DS:  # continuous mode has no guard pick to place
+        candidates = tuple(c for c in candidates if c["guard"] == "inline")
+    if use_cache:
+        hit = _load(path).get(key)
+        if hit and any(_tname(hit) == _tname(c) for c in candidates):
+            return dict(hit, cached=True)
+    timings = {_tname(c): _time_transport(tr, n, c, seed, reps, device) for c in candidates}
+    best = min(candidates, key=lambda c: timings[_tname(c)])
+    out = dict(block=best["block"], waves_per_simd=best["waves_per_simd"], guard=best["guard"], ms=timings)
+    if use_cache:
+        _store(path, key, out)
+    return dict(out, cached=False)
 
 
 def _load(path):
@@ -102,14 +187,8 @@ def tune_batch(items, seed=3, candidates=CANDIDATES, reps=2, device=0, use_cache
     best = min(candidates, key=lambda c: timings[_name(c)])
     out = dict(block=best["block"], pick_nodes=best["pick_nodes"], guard=best.get("guard"), ms=timings)
     if use_cache:
-        data = _load(path)
-        data[key] = out
-        os.makedirs(os.path.dirname(path), exist_ok=True)
-        tmp = path + f".{os.getpid()}.tmp"
-        with open(tmp, "w") as f:
-            json.dump(data, f, indent=1)
-        os.replace(tmp, path)
+        _store(path, key, out)
     return dict(out, cached=False)
 
 
-__all__ = ["CANDIDATES", "batch_key", "tune_batch"]
+__all__ = ["CANDIDATES", "TRANSPORT_CANDIDATES", "batch_key", "transport_key", "tune_batch", "tune_transport"]

@@ -68,6 +68,9 @@ class Transport:
         self.mode = self.cfg.mode
         self.tol_E, self.tol_W = tol_E, tol_W
         self.grids = grids
+        # launch shape used when a launch leaves block / waves_per_simd / guard
+        # at their defaults; set by tune()
+        self.tuned = None
         self._tables = C.c_void_p()
         ctx = context(device)
         if self.mode == _abi.MODE_DS:
@@ -116,6 +119,10 @@ class Transport:
         """Asynchronous launch on `stream` into caller-owned torch buffers.
         guard: None (library default: inline), "inline" or "deferred" -- where
         the DS FP64 guard pick runs (same records either way)."""
+        if self.tuned is not None and precision == _abi.PREC_FP32:
+            block = block or self.tuned["block"]
+            waves_per_simd = waves_per_simd or self.tuned["waves_per_simd"]
+            guard = guard or self.tuned["guard"]
         ctx = context(self.device)
         sb = None
         if scat_rows is not None:
@@ -127,6 +134,15 @@ class Transport:
         check(lib().mcd_forward(ctx, C.byref(self.cfg), self._tables, int(seed), int(config_id), int(first), int(n),
                                 els.data_ptr(), C.byref(sb) if sb is not None else None, counters.data_ptr(),
                                 C.byref(la), s), ctx)
+
+    def tune(self, n=1 << 22, **kw):
+        """Time the launch shapes of mcd.autotune.TRANSPORT_CANDIDATES on n
+        electrons of this configuration, keep the fastest for later launches
+        (records are the same for every shape) and return the tuner's report."""
+        from .autotune import tune_transport
+        res = tune_transport(self, n, **kw)
+        self.tuned = dict(block=res["block"], waves_per_simd=res["waves_per_simd"], guard=res["guard"])
+        return res
 
     def run(self, n, seed=1, first=0, config_id=0, scats=False, scat_max_electron=None, scat_capacity=None,
             precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None) -> RunResult:

@@ -273,3 +273,29 @@ def test_deferred_guard_equals_inline_guard(cu20, si10):
     rb = trs.run(512, seed=0, scats=True, guard="inline")
     assert ra.els.tobytes() == rb.els.tobytes()
     assert ra.scats.tobytes() == rb.scats.tobytes()
+
+
+def test_transport_autotuner_picks_a_shape_and_keeps_records(cu20, tmp_path):
+    """The single-configuration tuner times every candidate shape, caches the
+    fastest by configuration and electron count, and launches with the tuned
+    shape write the same records as the library default."""
+    from mcd import autotune
+    tr = cu20[0]
+    n = 1 << 16
+    ref = tr.run(n, seed=1)
+    path = str(tmp_path / "tt.json")
+    first = autotune.tune_transport(tr, n, reps=1, cache_path=path)
+    names = {autotune._tname(c) for c in autotune.TRANSPORT_CANDIDATES}
+    assert not first["cached"] and set(first["ms"]) == names
+    assert autotune._tname(first) in names
+    again = autotune.tune_transport(tr, n, reps=1, cache_path=path)
+    assert again["cached"] and autotune._tname(again) == autotune._tname(first)
+    assert autotune.transport_key(tr, n) != autotune.transport_key(tr, 2 * n)
+    for cand in autotune.TRANSPORT_CANDIDATES:
+        r = tr.run(n, seed=1, block=cand["block"], waves_per_simd=cand["waves_per_simd"], guard=cand["guard"])
+        assert r.els.tobytes() == ref.els.tobytes(), cand
+    try:
+        tr.tuned = dict(block=first["block"], waves_per_simd=first["waves_per_simd"], guard=first["guard"])
+        assert tr.run(n, seed=1).els.tobytes() == ref.els.tobytes()
+    finally:
+        tr.tuned = None
