@@ -360,7 +360,20 @@ __device__ __forceinline__ int f_pick_table(const float* __restrict__ nodes, con
   f_pick_eval<NCH>(nodes, P, E, o);
   mfp_ln2 = o.mfp_ln2;
   if (o.exact) return -1;
-  if (o.swap != (int)kMetaNoSwap && fabsf(o.pA - o.pB) <= P.delta) return -1;
+  if (o.swap != (int)kMetaNoSwap && fabsf(o.pA - o.pB) <= P.delta) {
+    // the order of the crossing pair (positions j, j+1) is uncertain, but the
+    // pick depends on it only when u falls in the pair's cumulative range
+    // [B[j-1], B[j+1]]; outside it (by more than delta) the count below
+    // decides, whichever way the pair is ordered
+    const int j = o.swap;
+    float lo = 0.f, hi = 1.f;
+#pragma unroll
+    for (int i = 0; i < NCH - 1; ++i) {
+      lo = (i == j - 1) ? o.B[i] : lo;
+      hi = (i == j + 1) ? o.B[i] : hi;
+    }
+    if (u >= lo - P.delta && u <= hi + P.delta) return -1;
+  }
   int cnt = 0;
   float gap = 1.f;
 #pragma unroll
