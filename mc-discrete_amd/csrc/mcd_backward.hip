@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
         const float Ec = E;
         int row;
         const int col = table_lookup(t, E, [Ec](float v) { return v < Ec; }, u_ang, row);
-        W = t.Ws[col];
+        W = ldg(t.Ws + col);
         const size_t cell = (size_t)row * (size_t)(t.nW - 1) + col;
         // the column pick's probability depends on the energy (likelihood ratio)
         const float gE = (ch == 1 ? A.m_score_e : A.g_score_e)[cell];

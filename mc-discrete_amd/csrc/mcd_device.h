@@ -111,6 +111,21 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
   return {c0, c1, c2, c3};
 }
 
+// Loads from global memory through a pointer whose address space the compiler
+// cannot see (table pointers arrive through LDS descriptors or by reference):
+// without the cast they become flat loads, which also wait on the LDS counter.
+template <class T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// Value of a uniform-indexed lane, as a wave-uniform (scalar) value.
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // ---------------------------------------------------------------------------
 // Work distribution of the persistent kernels.  A wavefront refills its idle
 // lanes from a reservation [next, end) of indices it has taken from a shared
@@ -141,8 +156,7 @@ __device__ __forceinline__ uint64_t refill_index(WaveQueue& q, unsigned long lon
     const int leader = __ffsll((unsigned long long)idle) - 1;
     unsigned long long base = 0;
     if (lane == leader) base = atomicAdd(counter, (unsigned long long)fetch);
-    const uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
-    const uint64_t blk = ((uint64_t)hi << 32) | lo;
+    const uint64_t blk = lane_u64(base, leader);
     idx = pfx < avail ? q.next + pfx : blk + (pfx - avail);
     q.next = blk + (nidle - avail);
     q.end = blk + fetch;
@@ -374,12 +388,16 @@ __device__ __forceinline__ int f_pick_table(const float* __restrict__ nodes, con
     }
     if (u >= lo - P.delta && u <= hi + P.delta) return -1;
   }
+  // count of boundaries below u from the sign of B[i] - u: exact (the
+  // difference is +0 only when they are equal, and a flushed subnormal keeps
+  // its sign)
   int cnt = 0;
   float gap = 1.f;
 #pragma unroll
   for (int i = 0; i < NCH - 1; ++i) {
-    cnt += (o.B[i] < u) ? 1 : 0;
-    gap = fminf(gap, fabsf(o.B[i] - u));
+    const float d = o.B[i] - u;
+    cnt += (int)(__float_as_uint(d) >> 31);
+    gap = fminf(gap, fabsf(d));
   }
   if (gap <= P.delta) return -1;
   return perm_label(o.perm_lo, o.perm_hi, cnt);
@@ -447,13 +465,13 @@ __device__ __forceinline__ int d_pick_ordered(double E, double u, const DevCfg& 
 __device__ __forceinline__ void guide_pair(const DevTable& t, const uint8_t* rec, float u, int& g0, int& g1) {
   if (t.narrow) {
     const int q = min((int)(u * (float)kGuideNarrow), kGuideNarrow - 1);
-    g0 = rec[kRowHeader + q];
-    g1 = rec[kRowHeader + q + 1];
+    g0 = ldg(rec + kRowHeader + q);
+    g1 = ldg(rec + kRowHeader + q + 1);
   } else {
     const int q = min((int)(u * (float)kGuideWide), kGuideWide - 1);
     const uint16_t* g = reinterpret_cast<const uint16_t*>(rec + kRowHeader) + q;
-    g0 = g[0];
-    g1 = g[1];
+    g0 = ldg(g);
+    g1 = ldg(g + 1);
   }
 }
 
@@ -466,14 +484,16 @@ __device__ __forceinline__ int table_lookup(const DevTable& t, float Eguess, Ene
   int i = (int)ceilf((Eguess - t.E_first) * t.inv_h);
   i = max(1, min(i, t.nE - 1));
   const uint8_t* rec = t.rows + (size_t)i * kRowBytes;
-  const float2 br = *reinterpret_cast<const float2*>(rec);
+  float2 br;
+  br.x = ldg(reinterpret_cast<const float*>(rec));
+  br.y = ldg(reinterpret_cast<const float*>(rec) + 1);
   int g0, g1;
   guide_pair(t, rec, u, g0, g1);
   if (!(es_lt_E(br.x) && !es_lt_E(br.y))) {
     int a = 0, b = t.nE;
     while (a < b) {
       const int mid = (a + b) >> 1;
-      if (es_lt_E(t.Es[mid])) a = mid + 1; else b = mid;
+      if (es_lt_E(ldg(t.Es + mid))) a = mid + 1; else b = mid;
     }
     i = min(a, t.nE - 1);
     rec = t.rows + (size_t)i * kRowBytes;
@@ -488,13 +508,14 @@ __device__ __forceinline__ int table_lookup(const DevTable& t, float Eguess, Ene
       // the CDF row is nondecreasing: col = g0 + #{j in [g0, g1) : cdf[j] < u};
       // four independent loads, clamped into [g0, g1)
       const int l = span - 1;
-      const float a0 = r[lo], a1 = r[lo + min(1, l)], a2 = r[lo + min(2, l)], a3 = r[lo + min(3, l)];
+      const float a0 = ldg(r + lo), a1 = ldg(r + lo + min(1, l)), a2 = ldg(r + lo + min(2, l)),
+                  a3 = ldg(r + lo + min(3, l));
       lo += (int)(a0 < u) + (int)(1 < span && a1 < u) + (int)(2 < span && a2 < u) + (int)(3 < span && a3 < u);
     } else {
       int hi = g1;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (r[mid] < u) lo = mid + 1; else hi = mid;
+        if (ldg(r + mid) < u) lo = mid + 1; else hi = mid;
       }
     }
   }

@@ -45,8 +45,7 @@ __device__ __forceinline__ uint64_t append_slot(bool want, unsigned long long* c
   int leader = __ffsll((unsigned long long)m) - 1;
   unsigned long long base = 0;
   if (lane == leader) base = atomicAdd(cursor, (unsigned long long)__popcll(m));
-  uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
-  base = ((uint64_t)hi << 32) | lo;
+  base = lane_u64(base, leader);
   uint64_t slot = base + __popcll(m & ((1ull << lane) - 1ull));
   return (want && slot < cap) ? slot : ~0ull;
 }
@@ -85,6 +84,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
   bool active = false, open = true;
   WaveQueue wq;
   uint64_t e = 0;
+  uint32_t ctr1 = 0;  // second Philox counter word of the lane's electron
   uint32_t step = 0, flags = 0;
   F2 E = {0.f, 0.f};
   float p[3] = {0.f, 0.f, 0.f}, pc[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
@@ -104,6 +104,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
             if constexpr (PHASE == 2) {
               const ParkRec& q = R.park[idx];
               e = q.e;
+              ctr1 = c.config_id | (uint32_t)(e >> 32) << 16;
               step = q.step;
               flags = q.flags;
               E.hi = q.Ehi; E.lo = q.Elo;
@@ -111,6 +112,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
               for (int j = 0; j < 3; ++j) { p[j] = q.p[j]; pc[j] = q.pc[j]; d[j] = q.d[j]; y[j] = q.y[j]; }
             } else {
               e = R.first + idx;
+              ctr1 = c.config_id | (uint32_t)(e >> 32) << 16;
               step = 0;
               flags = 0;
               E.hi = c.E0_hi; E.lo = c.E0_lo;
@@ -132,7 +134,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       active = false;
       continue;
     }
-    const U4 r = philox10((uint32_t)e, c.config_id | (uint32_t)(e >> 32) << 16, step, 0u, k0, k1);
+    const U4 r = philox10((uint32_t)e, ctr1, step, 0u, k0, k1);
     const F2 Epre = E;
     float l, Wh, Wl = 0.f, c2, s2, u_az;
     int ch = MCD_CH_RUTH;
@@ -149,8 +151,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
           const int leader = __ffsll((unsigned long long)m) - 1;
           unsigned long long base = 0;
           if (lane == leader) base = atomicAdd(R.park_count, (unsigned long long)__popcll(m));
-          const uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
-          const uint64_t slot = (((uint64_t)hi << 32) | lo) + __popcll(m & ((1ull << lane) - 1ull));
+          const uint64_t slot = lane_u64(base, leader) + __popcll(m & ((1ull << lane) - 1ull));
           ParkRec& q = R.park[slot];
           q.Ehi = E.hi; q.Elo = E.lo;
 #pragma unroll
@@ -195,7 +196,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
         const F2 Ec = E;
         int row;
         const int col = table_lookup(t, E.hi, [Ec](float v) { return f_lt_f2(v, Ec); }, u2, row);
-        Wh = t.Ws[col];
+        Wh = ldg(t.Ws + col);
         if (!f_lt_f2(Wh, E)) sflags |= MCD_FLAG_LOSS_GE_E;
         // binaryCollModel (:156-179): W <= (E - E_F)/2
         const F2 D = f2_sub(E, c.Ef2_hi, c.Ef2_lo);
