@@ -191,9 +191,10 @@ typedef struct mcd_launch {
 /* DS in FP32 can run the FP64 guard pick inline (one kernel) or deferred (two
  * kernels: the first has no FP64 code and parks an electron whose channel pick
  * falls in the guard band, the second resumes the parked electrons with the
- * FP64 pick inline).  Records are identical either way.  Defaults: mcd_forward
- * runs inline, mcd_batch_run deferred (the faster form of each on MI355X);
- * these flags select one explicitly.  Setting both is an error. */
+ * FP64 pick inline).  Records are identical either way.  Both mcd_forward and
+ * mcd_batch_run run inline unless MCD_LAUNCH_DEFER_GUARD is set (the faster
+ * form depends on the configuration; mcd/autotune.py measures it).  Setting
+ * both flags is an error. */
 #define MCD_LAUNCH_INLINE_GUARD 1
 #define MCD_LAUNCH_DEFER_GUARD 2
 

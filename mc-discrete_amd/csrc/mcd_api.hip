@@ -741,8 +741,8 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
   const int flags = launch ? launch->flags : 0;
   if ((flags & MCD_LAUNCH_INLINE_GUARD) && (flags & MCD_LAUNCH_DEFER_GUARD))
     return set_err(ctx, MCD_ERR_INVALID, "MCD_LAUNCH_INLINE_GUARD and MCD_LAUNCH_DEFER_GUARD both set");
-  // DS items defer the FP64 guard pick to a second phase unless asked not to
-  const bool deferred = b->has_ds && !(flags & MCD_LAUNCH_INLINE_GUARD);
+  // DS items defer the FP64 guard pick to a second phase only when asked to
+  const bool deferred = b->has_ds && (flags & MCD_LAUNCH_DEFER_GUARD);
   hipStream_t s = (hipStream_t)stream;
   if (!use_graph) {
     HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, deferred, block, s));

@@ -38,13 +38,13 @@ import os
 import time
 
 CANDIDATES = (
-    dict(block=256, pick_nodes=0, guard="deferred"),
-    dict(block=256, pick_nodes=128, guard="deferred"),
-    dict(block=256, pick_nodes=256, guard="deferred"),
-    dict(block=128, pick_nodes=0, guard="deferred"),
-    dict(block=128, pick_nodes=128, guard="deferred"),
-    dict(block=256, pick_nodes=0, guard="inline"),
+    dict(block=128, pick_nodes=128, guard="inline"),
     dict(block=256, pick_nodes=128, guard="inline"),
+    dict(block=256, pick_nodes=0, guard="inline"),
+    dict(block=256, pick_nodes=256, guard="inline"),
+    dict(block=128, pick_nodes=128, guard="deferred"),
+    dict(block=256, pick_nodes=128, guard="deferred"),
+    dict(block=256, pick_nodes=0, guard="deferred"),
 )
 
 

@@ -25,7 +25,7 @@ def small_batch():
     return cfgs, trs
 
 
-@pytest.mark.parametrize("use_graph,guard", [(False, None), (True, None), (True, "inline")])
+@pytest.mark.parametrize("use_graph,guard", [(False, None), (True, None), (True, "deferred")])
 def test_batched_equals_per_configuration_launches(small_batch, use_graph, guard):
     from mcd import _abi, engine
     cfgs, trs = small_batch
