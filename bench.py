@@ -179,8 +179,8 @@ def batched_ratio(reps=3):
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md and not launched", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for {a.gpus}); electron ranges would shard across GPUs, "
+              "but no multi-GPU run is launched", file=sys.stderr)
         sys.exit(2)
     import torch
 

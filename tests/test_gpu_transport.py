@@ -202,6 +202,28 @@ def test_split_launches_equal_one_launch(cu20):
     assert both.tobytes() == full.els.tobytes()
 
 
+@pytest.mark.parametrize("n", [1, 15, 17, 1000, (1 << 20) + 3])
+def test_every_electron_runs_once_at_any_count(cu20, n):
+    """Wavefronts take electron indices in blocks of 16 per atomic: for counts
+    that are not multiples of the block (or of a wavefront) every electron is
+    transported exactly once -- its record is written and equals the record of
+    the same electron in a larger launch (records are keyed by electron id)."""
+    tr = cu20[0]
+    for guard in ("inline", "deferred"):
+        r = tr.run(n, seed=7, first=3, guard=guard)
+        assert int(r.counters[_abi.CNT_ELECTRONS]) == n
+        outs = r.els["outcome"]
+        assert np.isin(outs, [_abi.OUT_BKS, _abi.OUT_ABS]).all(), guard
+        # an exit step that also ends below Emin records a second 'abs' outcome
+        # (saveOutcomes runs twice, electron.py:52-54,105-113)
+        o2 = r.els["outcome2"]
+        assert int(r.counters[_abi.CNT_BKS]) == int((outs == _abi.OUT_BKS).sum())
+        assert int(r.counters[_abi.CNT_ABS]) == int((outs == _abi.OUT_ABS).sum() + (o2 == _abi.OUT_ABS).sum())
+        big = tr.run(n + 40, seed=7, first=3, guard=guard)
+        assert big.els[:n].tobytes() == r.els.tobytes(), guard
+        assert int(r.counters[_abi.CNT_EVENTS]) == int(r.els["n_scatt"].astype(np.int64).sum()), guard
+
+
 def test_max_scatt_and_thin_film_outcomes():
     """maxScatt -> 'long' (singleScatter.py:19-23) and a finite thickness ->
     'trsm' (electron.py:72-81) against the oracle."""
