@@ -15,7 +15,7 @@ on the differentiation workload: Cu 20 keV -> 1 keV, s_tilt 70 deg, DS
 mcd_forward launch, backward = one mcd_backward launch (vector-Jacobian
 product for the yield) over the same electrons.
 
-Prints one JSON line.  With --gpus N > 1 it exits: this build runs on one GPU.
+Prints one JSON line.  With --gpus N > 1 it exits: the benchmark measures one GPU.
 """
 from __future__ import annotations
 

@@ -72,9 +72,9 @@ def _check_elastic(inputPar):
     el = inputPar.get("elastic", "Ruth_vanilla")
     if el != "Ruth_vanilla":
         raise NotImplementedError(f"elastic model {el!r}: only Ruth_vanilla is implemented "
-                                  "(Mott data and the Nigram/wDefl variants are not part of this build)")
+                                  "(Mott data and the Nigram/wDefl variants are not implemented)")
     if inputPar.get("diffMFP", False):
-        raise NotImplementedError("diffMFP: the diffraction channel is not part of this build")
+        raise NotImplementedError("diffMFP: the diffraction channel is not implemented")
 
 
 def genTables(inputPar):
