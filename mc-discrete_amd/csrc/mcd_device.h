@@ -41,13 +41,16 @@ constexpr uint32_t kMetaExact = 1u << 16;
 // quantile guide of row i's CDF: entry q = bisect_left(cdf_row, q / G), the
 // first column whose CDF >= q / G, so the column of u lies in
 // [guide[q], guide[q+1]] with q = floor(u G).  Tables with at most 255 loss
-// bins keep a uint8 guide with G = 512, wider ones a uint16 guide with G = 256.
+// bins keep a uint8 guide with G = 256, wider ones a uint16 guide with G = 128.
 // A lookup reads the guessed row's bracket and guide pair in one round trip
-// and, when the pair differs, up to four CDF entries in a second.
-constexpr int kGuideNarrow = 512;
-constexpr int kGuideWide = 256;
+// and, when the pair differs, up to four CDF entries in a second.  G = 256
+// keeps the two tables of a configuration with their records (~4.3 MB at
+// 2000 x 199) close to one XCD's 4 MB L2; G = 512 settled more lookups from
+// the guide alone but sent seven times the L2 misses to the fabric.
+constexpr int kGuideNarrow = 256;
+constexpr int kGuideWide = 128;
 constexpr int kRowHeader = 8;
-constexpr int kRowBytes = 528;
+constexpr int kRowBytes = 272;
 static_assert(kRowHeader + kGuideNarrow + 1 <= kRowBytes && kRowHeader + 2 * (kGuideWide + 1) <= kRowBytes,
               "row record too small");
 
