@@ -171,6 +171,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
   const float Emin = c.Emin_hi;
   const float Epl = c.Epl_hi, Ef = c.Ef2_hi;
   const float Jf = (float)c.J, kJ = (float)(c.bethe_kk * c.J);
+  const float inv_J = 1.f / Jf;
   const float* gup = hist;  // VJP: [dL/dY, dL/dH_0 ..]
 
   // per-lane accumulators: value and 3 slot derivatives
@@ -413,13 +414,13 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
         const float L = ln_f(c.bethe_c1_f * E);
         Sv = c.bethe_k1_f * iE * L;
         dSdE = -Sv * iE + c.bethe_k1_f * iE * iE;
-        dSdJ = -c.bethe_k1_f * iE / Jf;
+        dSdJ = -c.bethe_k1_f * iE * inv_J;
       } else if (c.bethe == MCD_BETHE_JL) {
         const float L = ln_f(c.bethe_c1_f * (E + kJ));
         const float ie2 = __builtin_amdgcn_rcpf(E + kJ);
         Sv = c.bethe_k1_f * iE * L;
         dSdE = -Sv * iE + c.bethe_k1_f * iE * ie2;
-        dSdJ = c.bethe_k1_f * iE * ((float)c.bethe_kk * ie2 - 1.f / Jf);
+        dSdJ = c.bethe_k1_f * iE * ((float)c.bethe_kk * ie2 - inv_J);
       } else {
         Sv = c.expl_k_f * iE * ln_f(E * c.expl_inv_Estar_f);
         dSdE = -Sv * iE + c.expl_k_f * iE * iE;
