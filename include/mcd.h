@@ -71,7 +71,9 @@ enum {
   MCD_FLAG_NAN_DIR = 1,    /* binaryCollModel returned NaN (W > (E-E_F)/2) */
   MCD_FLAG_LOSS_GE_E = 2,  /* energy loss >= energy (reference asserts, scattering.py:605,627,822) */
   MCD_FLAG_LONG_PATH = 4,  /* continuous-mode path >= 1e4 A (reference asserts, scattering.py:809) */
-  MCD_FLAG_FP64_PICK = 8   /* at least one channel pick was settled by the FP64 guard path */
+  MCD_FLAG_FP64_PICK = 8,  /* at least one channel pick was settled by the FP64 guard path */
+  MCD_FLAG_GUARD_TIMEOUT = 16 /* service guard: no reply within the wait bound; the pick is not
+                                 settled (counted in MCD_CNT_GUARD_TIMEOUTS; an error) */
 };
 
 /* Material constants (MC/tools/material.py:151-302) plus the derived
@@ -175,28 +177,33 @@ enum {
   MCD_CNT_FP64_PICKS = 6, /* picks settled by the FP64 guard */
   MCD_CNT_SCAT_ROWS = 7,  /* scatter rows requested (may exceed capacity) */
   MCD_CNT_QUEUE = 8,      /* internal: persistent-kernel work counter */
+  MCD_CNT_GUARD_TIMEOUTS = 9, /* service guard: picks left unsettled (must be 0) */
   MCD_NCOUNTERS = 16
 };
 
 /* Launch tuning (0 = library default). */
 typedef struct mcd_launch {
-  int32_t block;          /* threads per workgroup, multiple of 64, at most 256 */
+  int32_t block;          /* threads per workgroup, multiple of 64, at most 256 (service
+                             guard: 128 to 512, default 448) */
   int32_t waves_per_simd; /* occupancy variant of the one-kernel FP32 transport: resident
                              wavefronts per SIMD its register budget guarantees (1 = compiler's
-                             choice, 4 = default, 5, 6, 8) */
+                             choice, 4 = default, 5, 6, 8; service guard: 6 or 7 = default) */
   int32_t precision;      /* MCD_PREC_FP32 transport (default) or MCD_PREC_FP64 (all-FP64 run) */
   int32_t flags;          /* MCD_LAUNCH_* */
 } mcd_launch;
 
-/* DS in FP32 can run the FP64 guard pick inline (one kernel) or deferred (two
+/* DS in FP32 can run the FP64 guard pick inline (one kernel), deferred (two
  * kernels: the first has no FP64 code and parks an electron whose channel pick
  * falls in the guard band, the second resumes the parked electrons with the
- * FP64 pick inline).  Records are identical either way.  Both mcd_forward and
- * mcd_batch_run run inline unless MCD_LAUNCH_DEFER_GUARD is set (the faster
- * form depends on the configuration; mcd/autotune.py measures it).  Setting
- * both flags is an error. */
+ * FP64 pick inline) or served (mcd_forward only: one kernel in which wavefront
+ * 0 of each workgroup settles the guard-band picks that the other wavefronts
+ * post to it through LDS, so the transport code carries no FP64).  Records are
+ * identical in every form.  mcd_forward and mcd_batch_run run inline unless
+ * another form is asked for (the faster form depends on the configuration;
+ * mcd/autotune.py measures it).  Setting more than one flag is an error. */
 #define MCD_LAUNCH_INLINE_GUARD 1
 #define MCD_LAUNCH_DEFER_GUARD 2
+#define MCD_LAUNCH_SERVICE_GUARD 4
 
 typedef struct mcd_ctx mcd_ctx;
 typedef struct mcd_tables mcd_tables;

@@ -519,18 +519,36 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     R.row_max_electron = scats->max_electron;
     R.row_cursor = (unsigned long long*)&counters[MCD_CNT_SCAT_ROWS];
   }
-  int block = launch && launch->block > 0 ? launch->block : 256;
-  if (block % 64 || block > 256) return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64, at most 256", block);
-  int minw = launch && launch->waves_per_simd > 0 ? launch->waves_per_simd : default_minw(*cfg);
-  if (!valid_minw(minw)) return set_err(ctx, MCD_ERR_INVALID, "waves_per_simd %d not one of 1, 4, 5, 6, 8", minw);
-  const uint64_t want = (n + block - 1) / block;
   const int flags = launch ? launch->flags : 0;
-  if ((flags & MCD_LAUNCH_INLINE_GUARD) && (flags & MCD_LAUNCH_DEFER_GUARD))
-    return set_err(ctx, MCD_ERR_INVALID, "MCD_LAUNCH_INLINE_GUARD and MCD_LAUNCH_DEFER_GUARD both set");
-  // the FP64 guard pick runs inline unless deferral to a second kernel is asked
-  // for (DS, FP32; waves_per_simd applies to the one-kernel form only)
-  const bool deferred = cfg->mode == MCD_MODE_DS && precision == MCD_PREC_FP32 && (flags & MCD_LAUNCH_DEFER_GUARD);
-  if (deferred) {
+  const int guard_flags = flags & (MCD_LAUNCH_INLINE_GUARD | MCD_LAUNCH_DEFER_GUARD | MCD_LAUNCH_SERVICE_GUARD);
+  if (guard_flags & (guard_flags - 1))
+    return set_err(ctx, MCD_ERR_INVALID, "at most one of MCD_LAUNCH_INLINE_GUARD, _DEFER_GUARD, _SERVICE_GUARD");
+  // the FP64 guard pick runs inline unless another placement is asked for
+  // (DS, FP32 only; continuous mode has no guard)
+  const bool ds32 = cfg->mode == MCD_MODE_DS && precision == MCD_PREC_FP32;
+  const bool deferred = ds32 && (flags & MCD_LAUNCH_DEFER_GUARD);
+  const bool service = ds32 && (flags & MCD_LAUNCH_SERVICE_GUARD);
+  int block = launch && launch->block > 0 ? launch->block : (service ? 448 : 256);
+  if (service) {
+    if (block % 64 || block < 128 || block > 1024)
+      return set_err(ctx, MCD_ERR_INVALID, "service guard: block %d must be a multiple of 64 in [128, 1024]", block);
+  } else if (block % 64 || block > 256) {
+    return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64, at most 256", block);
+  }
+  int minw = launch && launch->waves_per_simd > 0 ? launch->waves_per_simd : (service ? 7 : default_minw(*cfg));
+  if (service ? !valid_svc_minw(minw) : !valid_minw(minw))
+    return set_err(ctx, MCD_ERR_INVALID, service ? "service guard: waves_per_simd %d not 6, 7 or 8"
+                                                 : "waves_per_simd %d not one of 1, 4, 5, 6, 8", minw);
+  const uint64_t want = (n + block - 1) / block;
+  if (service) {
+    // wavefront 0 of each workgroup serves the guard; the others carry electrons
+    const uint64_t want_svc = (n + (uint64_t)(block - 64) - 1) / (uint64_t)(block - 64);
+    uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)svc_blocks_per_cu(c, minw, block);
+    if (grid > want_svc) grid = want_svc;
+    if (grid < 1) grid = 1;
+    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
+    HIP_TRY(ctx, launch_transport_svc(c, R, minw, (int)grid, block, s));
+  } else if (deferred) {
     if (ctx->park_cap < n) {
       // the park buffer is reused across calls: wait for work that may still read it
       HIP_TRY(ctx, hipDeviceSynchronize());

@@ -81,6 +81,10 @@ int max_blocks_per_cu(const DevCfg& c, int precision, int minw, int block);
 // DS FP32 with the FP64 guard deferred: phase 1 parks guarded steps, phase 2 resumes them
 hipError_t launch_transport_deferred(const DevCfg& c, const RunArgs& R, int phase, int grid, int block, hipStream_t s);
 int deferred_blocks_per_cu(const DevCfg& c, int phase, int block);
+// DS FP32 with the FP64 guard served by wavefront 0 of each workgroup
+hipError_t launch_transport_svc(const DevCfg& c, const RunArgs& R, int minw, int grid, int block, hipStream_t s);
+int svc_blocks_per_cu(const DevCfg& c, int minw, int block);
+bool valid_svc_minw(int w);
 bool valid_minw(int w);
 hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsigned long long* counters, hipStream_t s);
 

@@ -62,6 +62,88 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const
 }
 
 // ---------------------------------------------------------------------------
+// Service guard (DS, FP32, one kernel).  Wavefront 0 of a workgroup is a
+// service wavefront: a transport wavefront posts the guard-band picks of a step
+// to its LDS mailbox and waits; the service wavefront settles them in FP64, one
+// of its lanes per posted lane, and posts the channels back.  The FP64 code is
+// on a path no transport wavefront executes, so the kernel keeps the register
+// budget of an FP64-free step without a second kernel or a park buffer.
+struct GuardSlot {
+  float Ehi, Elo, u;
+  uint32_t perm_lo, perm_hi;
+  int meta;  // posted: swap position | exact << 8; replied: the channel
+};
+struct GuardBox {
+  unsigned long long mask;  // posted lanes
+  uint32_t state;           // kBoxPosted, then kBoxReplied
+  uint32_t pad;
+  GuardSlot lane[64];
+};
+constexpr uint32_t kBoxPosted = 1u, kBoxReplied = 2u;
+// A transport wavefront waits at most this many s_sleep(1) polls (64 clocks
+// each, ~0.1 s in all) for a reply; past it the step takes the Rutherford
+// channel and is flagged MCD_FLAG_GUARD_TIMEOUT (counted as an error).
+constexpr uint32_t kGuardSpinCap = 1u << 22;
+
+__device__ __forceinline__ uint32_t box_load(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void box_store(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The channel order d_pick_ordered expects: a crossing cell's swap was
+// applied to pe.perm by the interpolation; undo it so the FP64 pick sees node
+// k's order and decides the pair exactly.
+template <int NCH>
+__device__ __forceinline__ void guard_perm(const PickEval<NCH>& pe, uint32_t& plo, uint32_t& phi) {
+  plo = pe.perm_lo;
+  phi = pe.perm_hi;
+  if (pe.swap != (int)kMetaNoSwap && pe.pB > pe.pA) {
+    uint64_t pm = ((uint64_t)phi << 32) | plo;
+    const uint64_t la = (pm >> (4 * pe.swap)) & 15u, lb = (pm >> (4 * (pe.swap + 1))) & 15u;
+    pm &= ~((uint64_t)0xFFu << (4 * pe.swap));
+    pm |= (lb << (4 * pe.swap)) | (la << (4 * (pe.swap + 1)));
+    plo = (uint32_t)pm;
+    phi = (uint32_t)(pm >> 32);
+  }
+}
+
+// The service wavefront's loop: answer posted mailboxes until every transport
+// wavefront of the workgroup has finished (each adds one to *done on exit;
+// a wavefront exits only after its last reply, so none is left waiting).
+// It also stops after kServiceIdleCap idle polls (~1 minute), so it ends even
+// if that count were never reached; a transport wavefront left waiting then
+// flags its step (kGuardSpinCap).
+constexpr uint32_t kServiceIdleCap = 1u << 30;
+
+template <int NCH>
+__device__ void guard_service(const DevCfg& c, GuardBox* boxes, int n_boxes, uint32_t* done) {
+  const int lane = threadIdx.x & 63;
+  uint32_t idle = 0;
+  while (idle < kServiceIdleCap) {
+    bool served = false;
+    for (int w = 0; w < n_boxes; ++w) {
+      GuardBox& b = boxes[w];
+      if (box_load(&b.state) != kBoxPosted) continue;
+      if ((b.mask >> lane) & 1ull) {
+        GuardSlot& g = b.lane[lane];
+        const int meta = g.meta;
+        g.meta = d_pick_ordered<NCH>((double)g.Ehi + (double)g.Elo, (double)g.u, c, g.perm_lo, g.perm_hi,
+                                     meta & 0xFF, (meta >> 8) & 1);
+      }
+      box_store(&b.state, kBoxReplied);
+      served = true;
+    }
+    if (!served) {
+      if (box_load(done) == (uint32_t)n_boxes) break;
+      __builtin_amdgcn_s_sleep(2);
+      ++idle;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP32 transport (DS and continuous).
 // The persistent loop of one wavefront over the electrons of one
 // configuration: refill idle lanes from the configuration's work counter,
@@ -72,11 +154,12 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const
 // of the step) and the lane takes the next electron.  PHASE 2: refill from the
 // parked electrons instead of new ones, guard inline.  Phases 1 + 2 give the
 // same records as phase 0: a parked step is redone from the same state and
-// Philox counter.
+// Philox counter.  PHASE 3 (DS): guard-band picks go to the workgroup's
+// service wavefront through `box`; same records as phase 0.
 template <int MODE, int NCH, int PHASE = 0>
 __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds,
                                                const DevTable* __restrict__ tabs, const RunArgs& R,
-                                               uint32_t& n_events, uint32_t& n_fp64) {
+                                               uint32_t& n_events, uint32_t& n_fp64, GuardBox* box = nullptr) {
   const int lane = threadIdx.x & 63;
   uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
   const uint64_t n_work = PHASE == 2 ? (uint64_t)__atomic_load_n(R.park_count, __ATOMIC_RELAXED) : R.n;
@@ -162,19 +245,40 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
           active = false;
           continue;
         }
+      } else if constexpr (PHASE == 3) {
+        if (__any(ch < 0)) {  // guard band: post to the service wavefront and wait
+          if (ch < 0) {
+            GuardSlot& g = box->lane[lane];
+            uint32_t plo, phi;
+            guard_perm<NCH>(pe, plo, phi);
+            g.Ehi = E.hi; g.Elo = E.lo; g.u = u0;
+            g.perm_lo = plo; g.perm_hi = phi;
+            g.meta = (pe.swap & 0xFF) | (pe.exact ? 0x100 : 0);
+          }
+          const uint64_t m = __ballot(ch < 0);
+          const int leader = __ffsll((unsigned long long)__ballot(true)) - 1;
+          if (lane == leader) {
+            box->mask = m;
+            box_store(&box->state, kBoxPosted);
+          }
+          uint32_t spins = 0;
+          while (box_load(&box->state) != kBoxReplied && ++spins < kGuardSpinCap) __builtin_amdgcn_s_sleep(1);
+          if (ch < 0) {
+            if (spins < kGuardSpinCap) {
+              ch = box->lane[lane].meta;
+              sflags |= MCD_FLAG_FP64_PICK;
+              n_fp64++;
+            } else {
+              ch = MCD_CH_RUTH;
+              sflags |= MCD_FLAG_GUARD_TIMEOUT;
+              atomicAdd(R.counters + MCD_CNT_GUARD_TIMEOUTS, 1ull);
+            }
+          }
+        }
       } else {
         if (ch < 0) {  // guard band: settle the pick in FP64
-          // a crossing cell's swap was applied to pe.perm by the interpolation; undo it
-          // so d_pick_ordered sees node k's order and decides the pair exactly
-          uint32_t plo = pe.perm_lo, phi = pe.perm_hi;
-          if (pe.swap != (int)kMetaNoSwap && pe.pB > pe.pA) {
-            uint64_t pm = ((uint64_t)phi << 32) | plo;
-            const uint64_t la = (pm >> (4 * pe.swap)) & 15u, lb = (pm >> (4 * (pe.swap + 1))) & 15u;
-            pm &= ~((uint64_t)0xFFu << (4 * pe.swap));
-            pm |= (lb << (4 * pe.swap)) | (la << (4 * (pe.swap + 1)));
-            plo = (uint32_t)pm;
-            phi = (uint32_t)(pm >> 32);
-          }
+          uint32_t plo, phi;
+          guard_perm<NCH>(pe, plo, phi);
           ch = d_pick_ordered<NCH>((double)E.hi + (double)E.lo, (double)u0, c, plo, phi, pe.swap, pe.exact);
           sflags |= MCD_FLAG_FP64_PICK;
           n_fp64++;
@@ -302,6 +406,48 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
   uint32_t n_events = 0, n_fp64 = 0;
   transport_loop<MODE, NCH, PHASE>(c, lds, tabs, R, n_events, n_fp64);
   const int lane = threadIdx.x & 63;
+  const uint64_t cnt[2] = {n_events, n_fp64};
+  wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
+  wave_add_counters(R.counters + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
+}
+
+// Service-guard kernel (DS, FP32).  LDS: the pick table, the two loss-table
+// descriptors, one GuardBox per transport wavefront, the finished-wavefront
+// count.  Wavefront 0 serves the guard picks, the others run the transport.
+// MINW: resident wavefronts per SIMD the register allocation must allow.
+__host__ __device__ constexpr size_t svc_box_offset(int nf) {
+  return ((size_t)tab_offset_floats(nf) * sizeof(float) + 2 * sizeof(DevTable) + 15) & ~(size_t)15;
+}
+static size_t svc_lds_bytes(int nf, int block) {
+  return svc_box_offset(nf) + (size_t)(block / 64 - 1) * sizeof(GuardBox) + 16;
+}
+
+template <int NCH, int MINW>
+__global__ __launch_bounds__(1024, MINW) void k_transport_svc(DevCfg c, RunArgs R) {
+  extern __shared__ float lds[];
+  const int nf = c.pick.n_nodes * c.pick.stride;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
+  DevTable* tabs = reinterpret_cast<DevTable*>(lds + tab_offset_floats(nf));
+  const int n_boxes = (int)(blockDim.x >> 6) - 1;
+  GuardBox* boxes = reinterpret_cast<GuardBox*>(reinterpret_cast<char*>(lds) + svc_box_offset(nf));
+  uint32_t* done = reinterpret_cast<uint32_t*>(boxes + n_boxes);
+  if (threadIdx.x == 0) {
+    tabs[0] = c.tm;
+    tabs[1] = c.tg;
+    *done = 0u;
+  }
+  if ((int)threadIdx.x < n_boxes) boxes[threadIdx.x].state = 0u;
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (w == 0) {
+    guard_service<NCH>(c, boxes, n_boxes, done);
+    return;
+  }
+  uint32_t n_events = 0, n_fp64 = 0;
+  transport_loop<MCD_MODE_DS, NCH, 3>(c, lds, tabs, R, n_events, n_fp64, boxes + (w - 1));
+  const int lane = threadIdx.x & 63;
+  // every lane is converged here; the service wavefront stops once all are counted
+  if (lane == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   const uint64_t cnt[2] = {n_events, n_fp64};
   wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
   wave_add_counters(R.counters + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
@@ -770,6 +916,61 @@ hipError_t launch_transport_deferred(const DevCfg& c, const RunArgs& R, int phas
   if (c.mode != MCD_MODE_DS) return hipErrorInvalidValue;
   return phase == 1 ? deferred_dispatch<1>(c, &R, grid, block, s, nullptr)
                     : deferred_dispatch<2>(c, &R, grid, block, s, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Service guard (DS, FP32): one kernel, wavefront 0 of each workgroup settles
+// the guard-band picks of the others.
+template <int NCH, int MINW>
+static hipError_t svc_nch(const DevCfg& c, const RunArgs* R, int grid, int block, size_t lds, hipStream_t s,
+                          int* occ) {
+  if (occ) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_svc<NCH, MINW>, block, lds) != hipSuccess)
+      nb = 1;
+    *occ = nb < 1 ? 1 : nb;
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL((k_transport_svc<NCH, MINW>), dim3(grid), dim3(block), lds, s, c, *R);
+  return hipGetLastError();
+}
+
+template <int MINW>
+static hipError_t svc_dispatch(const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s, int* occ) {
+  const size_t lds = svc_lds_bytes(c.pick.n_nodes * c.pick.stride, block);
+  switch (c.n_shells + 3) {
+    case 4: return svc_nch<4, MINW>(c, R, grid, block, lds, s, occ);
+    case 5: return svc_nch<5, MINW>(c, R, grid, block, lds, s, occ);
+    case 6: return svc_nch<6, MINW>(c, R, grid, block, lds, s, occ);
+    case 7: return svc_nch<7, MINW>(c, R, grid, block, lds, s, occ);
+    case 8: return svc_nch<8, MINW>(c, R, grid, block, lds, s, occ);
+    case 9: return svc_nch<9, MINW>(c, R, grid, block, lds, s, occ);
+    case 10: return svc_nch<10, MINW>(c, R, grid, block, lds, s, occ);
+    case 11: return svc_nch<11, MINW>(c, R, grid, block, lds, s, occ);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool valid_svc_minw(int w) { return w == 6 || w == 7 || w == 8; }
+
+static hipError_t svc_any(int minw, const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s, int* occ) {
+  switch (minw) {
+    case 6: return svc_dispatch<6>(c, R, grid, block, s, occ);
+    case 7: return svc_dispatch<7>(c, R, grid, block, s, occ);
+    case 8: return svc_dispatch<8>(c, R, grid, block, s, occ);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_transport_svc(const DevCfg& c, const RunArgs& R, int minw, int grid, int block, hipStream_t s) {
+  if (c.mode != MCD_MODE_DS || block < 128 || block > 1024 || block % 64) return hipErrorInvalidValue;
+  return svc_any(minw, c, &R, grid, block, s, nullptr);
+}
+
+int svc_blocks_per_cu(const DevCfg& c, int minw, int block) {
+  int nb = 1;
+  hipError_t e = svc_any(minw, c, nullptr, 0, block, 0, &nb);
+  return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
 int deferred_blocks_per_cu(const DevCfg& c, int phase, int block) {

@@ -14,11 +14,12 @@ BETHE_CLASSICAL, BETHE_JL, BETHE_EXPLICIT = 0, 1, 2
 OUT_NONE, OUT_ABS, OUT_BKS, OUT_TRSM, OUT_TRDF, OUT_LONG = range(6)
 OUTCOME_NAMES = {OUT_ABS: "abs", OUT_BKS: "bks", OUT_TRSM: "trsm", OUT_TRDF: "trdf", OUT_LONG: "long"}
 CH_RUTH, CH_MOLLER, CH_QUINN, CH_GRYZ0 = 0, 1, 2, 3
-FLAG_NAN_DIR, FLAG_LOSS_GE_E, FLAG_LONG_PATH, FLAG_FP64_PICK = 1, 2, 4, 8
+FLAG_NAN_DIR, FLAG_LOSS_GE_E, FLAG_LONG_PATH, FLAG_FP64_PICK, FLAG_GUARD_TIMEOUT = 1, 2, 4, 8, 16
 TABLE_MOLLER, TABLE_GRYZ = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 
-CNT_EVENTS, CNT_ELECTRONS, CNT_BKS, CNT_ABS, CNT_TRSM, CNT_LONG, CNT_FP64_PICKS, CNT_SCAT_ROWS, CNT_QUEUE = range(9)
+(CNT_EVENTS, CNT_ELECTRONS, CNT_BKS, CNT_ABS, CNT_TRSM, CNT_LONG, CNT_FP64_PICKS, CNT_SCAT_ROWS, CNT_QUEUE,
+ CNT_GUARD_TIMEOUTS) = range(10)
 NCOUNTERS = 16
 GRAD_WC, GRAD_TILT, GRAD_J, GRAD_LNRHO = range(4)
 NGRAD = 4
@@ -70,6 +71,7 @@ class Launch(C.Structure):
 
 LAUNCH_INLINE_GUARD = 1
 LAUNCH_DEFER_GUARD = 2
+LAUNCH_SERVICE_GUARD = 4
 
 
 class BwdOpts(C.Structure):

@@ -2,10 +2,12 @@
 batched runs (mcd_batch_*).
 
 Single configurations (``tune_transport``): ``block`` (threads per workgroup),
-``waves_per_simd`` (the register budget of the one-kernel transport: 1 leaves
-the allocation to the compiler, 4 and 5 cap it for more resident wavefronts)
-and ``guard`` (the FP64 guard pick inline, or deferred to a second kernel whose
-first phase carries no FP64 code).  Records are identical for every shape.
+``waves_per_simd`` (the register budget: the resident wavefronts per SIMD it
+must allow) and ``guard``, where a DS step whose channel pick falls in the
+guard band gets its FP64 pick: inline in the transport code, deferred to a
+second kernel whose first phase carries no FP64 code, or served by wavefront 0
+of each workgroup through LDS (one kernel whose transport code carries no
+FP64).  Records are identical for every shape.
 
 Batched runs (``tune_batch``):
 
@@ -55,10 +57,11 @@ def _name(c):
 TRANSPORT_CANDIDATES = (
     dict(block=256, waves_per_simd=4, guard="inline"),
     dict(block=128, waves_per_simd=4, guard="inline"),
-    dict(block=256, waves_per_simd=1, guard="inline"),
     dict(block=256, waves_per_simd=5, guard="inline"),
     dict(block=256, waves_per_simd=0, guard="deferred"),
-    dict(block=128, waves_per_simd=0, guard="deferred"),
+    dict(block=512, waves_per_simd=8, guard="service"),
+    dict(block=1024, waves_per_simd=8, guard="service"),
+    dict(block=512, waves_per_simd=7, guard="service"),
 )
 
 
@@ -127,7 +130,8 @@ def tune_transport(tr, n=1 << 22, seed=1, candidates=TRANSPORT_CANDIDATES, reps=
 
     Returns {"block", "waves_per_simd", "guard", "ms": {candidate: ms}, "cached": bool}."""
     path = cache_path or _cache_path("transport_tune.json")
-    key = transport_key(tr, n)
+    # the key names the candidate set too, so a cached choice is one of today's
+    key = transport_key(tr, n) + hashlib.sha1(",".join(_tname(c) for c in candidates).encode()).hexdigest()[:8]
     from . import _abi
     if tr.mode != _abi.MODE_DS:  # continuous mode has no guard pick to place
         candidates = tuple(c for c in candidates if c["guard"] == "inline")

@@ -45,14 +45,17 @@ class RunResult:
 
 
 def guard_flags(guard):
-    """mcd_launch.flags for a guard placement: None, "inline" or "deferred"."""
+    """mcd_launch.flags for a guard placement: None, "inline", "deferred" or
+    "service" (the last for single-configuration launches only)."""
     if guard is None:
         return 0
     if guard == "inline":
         return _abi.LAUNCH_INLINE_GUARD
     if guard == "deferred":
         return _abi.LAUNCH_DEFER_GUARD
-    raise ValueError(f"guard must be None, 'inline' or 'deferred', not {guard!r}")
+    if guard == "service":
+        return _abi.LAUNCH_SERVICE_GUARD
+    raise ValueError(f"guard must be None, 'inline', 'deferred' or 'service', not {guard!r}")
 
 
 class Transport:
@@ -162,6 +165,9 @@ class Transport:
                     waves_per_simd=waves_per_simd, guard=guard)
         torch.cuda.synchronize(dev)
         counters = cnt.cpu().numpy().view(np.uint64)
+        if counters[_abi.CNT_GUARD_TIMEOUTS]:
+            raise RuntimeError(f"{int(counters[_abi.CNT_GUARD_TIMEOUTS])} guard picks were not settled "
+                               "(service wavefront did not reply within the wait bound)")
         e = els.cpu().numpy()[: n * _abi.ELECTRON_DTYPE.itemsize].view(_abi.ELECTRON_DTYPE)
         srows = None
         if rows is not None:

@@ -209,7 +209,7 @@ def test_every_electron_runs_once_at_any_count(cu20, n):
     transported exactly once -- its record is written and equals the record of
     the same electron in a larger launch (records are keyed by electron id)."""
     tr = cu20[0]
-    for guard in ("inline", "deferred"):
+    for guard in ("inline", "deferred", "service"):
         r = tr.run(n, seed=7, first=3, guard=guard)
         assert int(r.counters[_abi.CNT_ELECTRONS]) == n
         outs = r.els["outcome"]
@@ -278,23 +278,73 @@ def test_reference_statistical_checks_on_gpu_output(si10):
     assert watson_u2(theory, mc) < 0.152
 
 
-def test_deferred_guard_equals_inline_guard(cu20, si10):
+@pytest.mark.parametrize("guard,shape", [
+    ("deferred", {}),
+    ("service", {}),
+    ("service", dict(block=128, waves_per_simd=8)),
+    ("service", dict(block=1024, waves_per_simd=8)),
+    ("service", dict(block=512, waves_per_simd=6)),
+])
+def test_guard_forms_equal_inline_guard(cu20, si10, guard, shape):
     """The two-kernel DS form (FP64 guard picks parked and resumed in a second
-    kernel) writes the same records, scatter rows and counters as the
+    kernel) and the served form (guard picks posted through LDS to wavefront 0
+    of the workgroup, at any workgroup size: one transport wavefront up to
+    fifteen) write the same records, scatter rows and counters as the
     one-kernel form with the guard inline."""
     tr = cu20[0]
     n = 1 << 20
-    a = tr.run(n, seed=1, guard="deferred")
+    a = tr.run(n, seed=1, guard=guard, **shape)
     b = tr.run(n, seed=1)
     assert a.els.tobytes() == b.els.tobytes()
     for k in (_abi.CNT_EVENTS, _abi.CNT_BKS, _abi.CNT_ABS, _abi.CNT_FP64_PICKS):
         assert int(a.counters[k]) == int(b.counters[k]), k
-    assert int(a.counters[_abi.CNT_FP64_PICKS]) > 0  # the resume kernel did run
+    assert int(a.counters[_abi.CNT_FP64_PICKS]) > 0  # guard picks were settled in FP64
+    assert int(a.counters[_abi.CNT_GUARD_TIMEOUTS]) == 0
     trs = si10[0]
-    ra = trs.run(512, seed=0, scats=True, guard="deferred")
+    ra = trs.run(512, seed=0, scats=True, guard=guard, **shape)
     rb = trs.run(512, seed=0, scats=True, guard="inline")
     assert ra.els.tobytes() == rb.els.tobytes()
     assert ra.scats.tobytes() == rb.scats.tobytes()
+
+
+@pytest.mark.parametrize("material,E0,Emin", [("Au", 30000.0, 1000.0), ("Al", 5000.0, 500.0)])
+def test_service_guard_other_channel_counts(material, E0, Emin):
+    """The served guard at 11 channels (Au, four Gryzinski shells listed plus
+    Moller, Rutherford, Quinn) and at 6 (Al) writes the inline form's records."""
+    from mcd import engine
+    tr = engine.Transport(material, E0, Emin, Wc=100.0, s_tilt=70.0, tol_E=1e-2, tol_W=1e-2)
+    a = tr.run(1 << 17, seed=4, guard="service")
+    b = tr.run(1 << 17, seed=4, guard="inline")
+    assert a.els.tobytes() == b.els.tobytes()
+    assert int(a.counters[_abi.CNT_FP64_PICKS]) == int(b.counters[_abi.CNT_FP64_PICKS])
+
+
+def test_launch_shape_errors_are_reported(cu20):
+    """The C-ABI rejects two guard placements at once and out-of-range shapes
+    with MCD_ERR_INVALID before launching anything."""
+    import ctypes as C
+    import torch
+    from mcd import engine
+    from mcd._lib import MCDError, check, lib
+    tr = cu20[0]
+    dev = torch.device("cuda", 0)
+    els = torch.empty(64 * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    ctx = engine.context(0)
+    s = torch.cuda.current_stream(0).cuda_stream
+    bad = [(0, 0, _abi.LAUNCH_INLINE_GUARD | _abi.LAUNCH_SERVICE_GUARD),
+           (0, 0, _abi.LAUNCH_DEFER_GUARD | _abi.LAUNCH_SERVICE_GUARD),
+           (1088, 0, _abi.LAUNCH_SERVICE_GUARD),
+           (64, 0, _abi.LAUNCH_SERVICE_GUARD),
+           (512, 5, _abi.LAUNCH_SERVICE_GUARD),
+           (512, 0, _abi.LAUNCH_INLINE_GUARD)]
+    for block, w, flags in bad:
+        la = _abi.Launch(block, w, _abi.PREC_FP32, flags)
+        rc = lib().mcd_forward(ctx, C.byref(tr.cfg), tr._tables, 1, 0, 0, 64, els.data_ptr(), None,
+                               cnt.data_ptr(), C.byref(la), s)
+        with pytest.raises(MCDError):
+            check(rc, ctx)
+    torch.cuda.synchronize(dev)
 
 
 def test_transport_autotuner_picks_a_shape_and_keeps_records(cu20, tmp_path):

@@ -72,6 +72,22 @@ def test_library_exports_every_declared_symbol():
     assert L.mcd_version() == 1
 
 
+def test_python_constants_match_the_header():
+    """The ctypes mirror (_abi) uses the header's values for launch flags,
+    record flags and counter slots."""
+    from mcd import _abi
+    hdr = open(os.path.join(ROOT, "include", "mcd.h")).read()
+    vals = {k: int(v) for k, v in re.findall(r"#define (MCD_LAUNCH_\w+) (\d+)", hdr)}
+    vals.update({k: int(v) for k, v in re.findall(r"\b(MCD_(?:FLAG|CNT)_\w+) = (\d+)", hdr)})
+    want = {"MCD_LAUNCH_INLINE_GUARD": _abi.LAUNCH_INLINE_GUARD, "MCD_LAUNCH_DEFER_GUARD": _abi.LAUNCH_DEFER_GUARD,
+            "MCD_LAUNCH_SERVICE_GUARD": _abi.LAUNCH_SERVICE_GUARD, "MCD_FLAG_FP64_PICK": _abi.FLAG_FP64_PICK,
+            "MCD_FLAG_GUARD_TIMEOUT": _abi.FLAG_GUARD_TIMEOUT, "MCD_CNT_EVENTS": _abi.CNT_EVENTS,
+            "MCD_CNT_FP64_PICKS": _abi.CNT_FP64_PICKS, "MCD_CNT_QUEUE": _abi.CNT_QUEUE,
+            "MCD_CNT_GUARD_TIMEOUTS": _abi.CNT_GUARD_TIMEOUTS}
+    for k, v in want.items():
+        assert vals[k] == v, k
+
+
 def test_unsupported_reference_options_raise():
     from mcd.multiScatter import genTables
     with pytest.raises(NotImplementedError):
