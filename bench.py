@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=0)
-    ap.add_argument("--guard", choices=("inline", "deferred"), default=None,
+    ap.add_argument("--guard", choices=("inline", "deferred", "service"), default=None,
                     help="fix the FP64 guard placement (with --block/--waves-per-simd: skips the autotuner)")
     ap.add_argument("--no-backward", action="store_true", help="skip the backward/forward ratio")
     ap.add_argument("--bwd-electrons", type=int, default=1 << 24)
@@ -96,7 +96,8 @@ def cpu_baselines(tr, seconds, seed):
 
 def backward_ratio(n, reps=3):
     """Forward and backward kernel times (HIP events on the launch stream) on
-    the differentiation workload, DS and continuous JL."""
+    the differentiation workload, DS and continuous JL; the forward runs at
+    the launch shape the autotuner picks for that workload."""
     import torch
 
     from mcd import _abi
@@ -113,6 +114,7 @@ def backward_ratio(n, reps=3):
     for key, mode in (("DS", "DS"), ("cont_JL", "cont")):
         spec = BackscatterSpec(mode=mode, n=n, seed=4)
         tr = spec.transport(spec.params().tolist())
+        shape = tr.tune(n, seed=4, reps=1)
         tr.launch(n, els, cnt, seed=4)
         tr.vjp(n, g, seed=4, out=acc)
         torch.cuda.synchronize(dev)
@@ -129,7 +131,8 @@ def backward_ratio(n, reps=3):
             tf.append(e[0].elapsed_time(e[1]))
             tb.append(e[2].elapsed_time(e[3]))
         f, b = min(tf), min(tb)
-        out[key] = {"forward_ms": f, "backward_ms": b, "ratio": b / f}
+        out[key] = {"forward_ms": f, "backward_ms": b, "ratio": b / f,
+                    "forward_shape": {k: shape[k] for k in ("block", "waves_per_simd", "guard")}}
     out["ratio_max"] = max(out["DS"]["ratio"], out["cont_JL"]["ratio"])
     return out
 

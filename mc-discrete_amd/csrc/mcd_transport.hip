@@ -17,7 +17,22 @@ namespace mcd {
 // LDS of the one-configuration kernels: the pick table (nf floats), then the
 // two loss-table descriptors at a 16-byte aligned offset.
 __host__ __device__ constexpr int tab_offset_floats(int nf) { return (nf + 3) & ~3; }
-static size_t single_lds_bytes(int nf) { return (size_t)tab_offset_floats(nf) * sizeof(float) + 2 * sizeof(DevTable); }
+static size_t single_lds_bytes(const DevCfg& c) {
+  return (size_t)tab_offset_floats(c.pick.n_nodes * c.pick.stride) * sizeof(float) + 2 * sizeof(DevTable);
+}
+
+// Stage the pick table and the two loss-table descriptors (Moller,
+// Gryzinski) that a lane selects by its channel.
+__device__ __forceinline__ DevTable* stage_single(const DevCfg& c, float* lds) {
+  const int nf = c.pick.n_nodes * c.pick.stride;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
+  DevTable* tabs = reinterpret_cast<DevTable*>(lds + tab_offset_floats(nf));
+  if (threadIdx.x == 0) {
+    tabs[0] = c.tm;
+    tabs[1] = c.tg;
+  }
+  return tabs;
+}
 
 // The record is written once, when the trajectory ends, as nine dword
 // stores marked non-temporal: records of one wavefront land in different
@@ -76,12 +91,12 @@ struct GuardSlot {
 struct GuardBox {
   unsigned long long mask;  // posted lanes
   uint32_t state;           // kBoxPosted, then kBoxReplied
-  uint32_t pad;
+  uint32_t bit;             // this mailbox's bit in the workgroup's pending word
   GuardSlot lane[64];
 };
 constexpr uint32_t kBoxPosted = 1u, kBoxReplied = 2u;
-// A transport wavefront waits at most this many s_sleep(1) polls (64 clocks
-// each, ~0.1 s in all) for a reply; past it the step takes the Rutherford
+// A transport wavefront waits at most this many s_sleep(2) polls (128 clocks
+// each, ~0.2 s in all) for a reply; past it the step takes the Rutherford
 // channel and is flagged MCD_FLAG_GUARD_TIMEOUT (counted as an error).
 constexpr uint32_t kGuardSpinCap = 1u << 22;
 
@@ -112,19 +127,35 @@ __device__ __forceinline__ void guard_perm(const PickEval<NCH>& pe, uint32_t& pl
 // The service wavefront's loop: answer posted mailboxes until every transport
 // wavefront of the workgroup has finished (each adds one to *done on exit;
 // a wavefront exits only after its last reply, so none is left waiting).
-// It also stops after kServiceIdleCap idle polls (~1 minute), so it ends even
-// if that count were never reached; a transport wavefront left waiting then
+// A posting wavefront sets its mailbox's bit in *pending, so an idle poll is
+// one LDS read between sleeps of 512 clocks (the idle service wavefront takes
+// few issue slots from the transport wavefronts on its SIMD).  It also stops
+// after kServiceIdleCap idle polls (~4 minutes), so it ends even if the
+// finished count were never reached; a transport wavefront left waiting then
 // flags its step (kGuardSpinCap).
 constexpr uint32_t kServiceIdleCap = 1u << 30;
 
 template <int NCH>
-__device__ void guard_service(const DevCfg& c, GuardBox* boxes, int n_boxes, uint32_t* done) {
+__device__ void guard_service(const DevCfg& c, GuardBox* boxes, uint32_t* pending, uint32_t* done, int n_boxes) {
   const int lane = threadIdx.x & 63;
+  const uint32_t nb = (uint32_t)__builtin_amdgcn_readfirstlane(n_boxes);
+  // a waiting transport wavefront is blocked until the reply: serve first
+  __builtin_amdgcn_s_setprio(3);
   uint32_t idle = 0;
   while (idle < kServiceIdleCap) {
-    bool served = false;
-    for (int w = 0; w < n_boxes; ++w) {
-      GuardBox& b = boxes[w];
+    if (__builtin_amdgcn_readfirstlane(box_load(pending)) == 0u) {
+      if (__builtin_amdgcn_readfirstlane(box_load(done)) == nb) break;
+      __builtin_amdgcn_s_sleep(8);
+      ++idle;
+      continue;
+    }
+    // all 64 lanes are active here: lane 0 takes the pending set for the wavefront
+    uint32_t q = 0;
+    if (lane == 0) q = __hip_atomic_exchange(pending, 0u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    q = __builtin_amdgcn_readfirstlane(q);
+    while (q) {
+      GuardBox& b = boxes[__builtin_ctz(q)];
+      q &= q - 1u;
       if (box_load(&b.state) != kBoxPosted) continue;
       if ((b.mask >> lane) & 1ull) {
         GuardSlot& g = b.lane[lane];
@@ -133,12 +164,6 @@ __device__ void guard_service(const DevCfg& c, GuardBox* boxes, int n_boxes, uin
                                      meta & 0xFF, (meta >> 8) & 1);
       }
       box_store(&b.state, kBoxReplied);
-      served = true;
-    }
-    if (!served) {
-      if (box_load(done) == (uint32_t)n_boxes) break;
-      __builtin_amdgcn_s_sleep(2);
-      ++idle;
     }
   }
 }
@@ -159,7 +184,8 @@ __device__ void guard_service(const DevCfg& c, GuardBox* boxes, int n_boxes, uin
 template <int MODE, int NCH, int PHASE = 0>
 __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds,
                                                const DevTable* __restrict__ tabs, const RunArgs& R,
-                                               uint32_t& n_events, uint32_t& n_fp64, GuardBox* box = nullptr) {
+                                               uint32_t& n_events, uint32_t& n_fp64, GuardBox* box = nullptr,
+                                               uint32_t* pending = nullptr) {
   const int lane = threadIdx.x & 63;
   uint32_t k0 = (uint32_t)R.seed, k1 = (uint32_t)(R.seed >> 32);
   const uint64_t n_work = PHASE == 2 ? (uint64_t)__atomic_load_n(R.park_count, __ATOMIC_RELAXED) : R.n;
@@ -209,6 +235,17 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       }
     }
     if (!__any(active)) break;
+    if constexpr (PHASE == 3) {
+      // Past the end of the work counter the launch's remaining time is set by
+      // the longest remaining trajectories, which are the highest-energy
+      // electrons: wavefronts holding one issue before the others on their SIMD.
+      if (!open) {
+        const float span = c.E0_hi - c.Emin_hi;
+        if (__any(active && E.hi > c.Emin_hi + 0.5f * span)) __builtin_amdgcn_s_setprio(3);
+        else if (__any(active && E.hi > c.Emin_hi + 0.2f * span)) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(1);
+      }
+    }
     if (!active) continue;
 
     // ---- one loop iteration of trajectory_DS / trajectory_cont_cl ----
@@ -260,9 +297,10 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
           if (lane == leader) {
             box->mask = m;
             box_store(&box->state, kBoxPosted);
+            __hip_atomic_fetch_or(pending, box->bit, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
           uint32_t spins = 0;
-          while (box_load(&box->state) != kBoxReplied && ++spins < kGuardSpinCap) __builtin_amdgcn_s_sleep(1);
+          while (box_load(&box->state) != kBoxReplied && ++spins < kGuardSpinCap) __builtin_amdgcn_s_sleep(2);
           if (ch < 0) {
             if (spins < kGuardSpinCap) {
               ch = box->lane[lane].meta;
@@ -392,15 +430,7 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
   extern __shared__ float lds[];
   DevTable* tabs = nullptr;
   if (MODE == MCD_MODE_DS) {
-    // pick table, then the two loss-table descriptors (Moller, Gryzinski) that
-    // a lane selects by its channel
-    const int nf = c.pick.n_nodes * c.pick.stride;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
-    tabs = reinterpret_cast<DevTable*>(lds + tab_offset_floats(nf));
-    if (threadIdx.x == 0) {
-      tabs[0] = c.tm;
-      tabs[1] = c.tg;
-    }
+    tabs = stage_single(c, lds);
     __syncthreads();
   }
   uint32_t n_events = 0, n_fp64 = 0;
@@ -413,38 +443,40 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
 
 // Service-guard kernel (DS, FP32).  LDS: the pick table, the two loss-table
 // descriptors, one GuardBox per transport wavefront, the finished-wavefront
-// count.  Wavefront 0 serves the guard picks, the others run the transport.
+// count and the pending-mailbox bits.  Wavefront 0 serves the guard picks, the others run the transport.
 // MINW: resident wavefronts per SIMD the register allocation must allow.
-__host__ __device__ constexpr size_t svc_box_offset(int nf) {
-  return ((size_t)tab_offset_floats(nf) * sizeof(float) + 2 * sizeof(DevTable) + 15) & ~(size_t)15;
+__host__ __device__ inline size_t svc_box_offset(const DevCfg& c) {
+  return ((size_t)tab_offset_floats(c.pick.n_nodes * c.pick.stride) * sizeof(float) + 2 * sizeof(DevTable) + 15) &
+         ~(size_t)15;
 }
-static size_t svc_lds_bytes(int nf, int block) {
-  return svc_box_offset(nf) + (size_t)(block / 64 - 1) * sizeof(GuardBox) + 16;
+static size_t svc_lds_bytes(const DevCfg& c, int block) {
+  return svc_box_offset(c) + (size_t)(block / 64 - 1) * sizeof(GuardBox) + 16;
 }
 
 template <int NCH, int MINW>
 __global__ __launch_bounds__(1024, MINW) void k_transport_svc(DevCfg c, RunArgs R) {
   extern __shared__ float lds[];
-  const int nf = c.pick.n_nodes * c.pick.stride;
-  for (int i = threadIdx.x; i < nf; i += blockDim.x) lds[i] = c.pick.nodes[i];
-  DevTable* tabs = reinterpret_cast<DevTable*>(lds + tab_offset_floats(nf));
+  DevTable* tabs = stage_single(c, lds);
   const int n_boxes = (int)(blockDim.x >> 6) - 1;
-  GuardBox* boxes = reinterpret_cast<GuardBox*>(reinterpret_cast<char*>(lds) + svc_box_offset(nf));
+  GuardBox* boxes = reinterpret_cast<GuardBox*>(reinterpret_cast<char*>(lds) + svc_box_offset(c));
   uint32_t* done = reinterpret_cast<uint32_t*>(boxes + n_boxes);
+  uint32_t* pending = done + 1;
   if (threadIdx.x == 0) {
-    tabs[0] = c.tm;
-    tabs[1] = c.tg;
     *done = 0u;
+    *pending = 0u;
   }
-  if ((int)threadIdx.x < n_boxes) boxes[threadIdx.x].state = 0u;
+  if ((int)threadIdx.x < n_boxes) {
+    boxes[threadIdx.x].state = 0u;
+    boxes[threadIdx.x].bit = 1u << threadIdx.x;
+  }
   __syncthreads();
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (w == 0) {
-    guard_service<NCH>(c, boxes, n_boxes, done);
+    guard_service<NCH>(c, boxes, pending, done, n_boxes);
     return;
   }
   uint32_t n_events = 0, n_fp64 = 0;
-  transport_loop<MCD_MODE_DS, NCH, 3>(c, lds, tabs, R, n_events, n_fp64, boxes + (w - 1));
+  transport_loop<MCD_MODE_DS, NCH, 3>(c, lds, tabs, R, n_events, n_fp64, boxes + (w - 1), pending);
   const int lane = threadIdx.x & 63;
   // every lane is converged here; the service wavefront stops once all are counted
   if (lane == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -836,7 +868,7 @@ static hipError_t dispatch_w(int minw, const DevCfg& c, const RunArgs* R, int gr
 static hipError_t dispatch_f32(int minw, const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s,
                                int* occ) {
   if (c.mode == MCD_MODE_CONT) return dispatch_w<MCD_MODE_CONT, 4>(minw, c, R, grid, block, 0, s, occ);
-  const size_t lds = single_lds_bytes(c.pick.n_nodes * c.pick.stride);
+  const size_t lds = single_lds_bytes(c);
   switch (c.n_shells + 3) {
     case 4: return dispatch_w<MCD_MODE_DS, 4>(minw, c, R, grid, block, lds, s, occ);
     case 5: return dispatch_w<MCD_MODE_DS, 5>(minw, c, R, grid, block, lds, s, occ);
@@ -897,7 +929,7 @@ static hipError_t deferred_nch(const DevCfg& c, const RunArgs* R, int grid, int 
 
 template <int PHASE>
 static hipError_t deferred_dispatch(const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s, int* occ) {
-  const size_t lds = single_lds_bytes(c.pick.n_nodes * c.pick.stride);
+  const size_t lds = single_lds_bytes(c);
   switch (c.n_shells + 3) {
     case 4: return deferred_nch<4, PHASE>(c, R, grid, block, lds, s, occ);
     case 5: return deferred_nch<5, PHASE>(c, R, grid, block, lds, s, occ);
@@ -937,7 +969,7 @@ static hipError_t svc_nch(const DevCfg& c, const RunArgs* R, int grid, int block
 
 template <int MINW>
 static hipError_t svc_dispatch(const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s, int* occ) {
-  const size_t lds = svc_lds_bytes(c.pick.n_nodes * c.pick.stride, block);
+  const size_t lds = svc_lds_bytes(c, block);
   switch (c.n_shells + 3) {
     case 4: return svc_nch<4, MINW>(c, R, grid, block, lds, s, occ);
     case 5: return svc_nch<5, MINW>(c, R, grid, block, lds, s, occ);
