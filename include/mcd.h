@@ -183,11 +183,11 @@ enum {
 
 /* Launch tuning (0 = library default). */
 typedef struct mcd_launch {
-  int32_t block;          /* threads per workgroup, multiple of 64, at most 256 (service
-                             guard: 128 to 512, default 448) */
+  int32_t block;          /* threads per workgroup, multiple of 64, at most 256 (served
+                             guard: 128 to 1024, default 1024) */
   int32_t waves_per_simd; /* occupancy variant of the one-kernel FP32 transport: resident
                              wavefronts per SIMD its register budget guarantees (1 = compiler's
-                             choice, 4 = default, 5, 6, 8; service guard: 6 or 7 = default) */
+                             choice, 4 = default, 5, 6, 8; served guard: 6, 7 or 8 = default) */
   int32_t precision;      /* MCD_PREC_FP32 transport (default) or MCD_PREC_FP64 (all-FP64 run) */
   int32_t flags;          /* MCD_LAUNCH_* */
 } mcd_launch;
@@ -198,9 +198,11 @@ typedef struct mcd_launch {
  * FP64 pick inline) or served (mcd_forward only: one kernel in which wavefront
  * 0 of each workgroup settles the guard-band picks that the other wavefronts
  * post to it through LDS, so the transport code carries no FP64).  Records are
- * identical in every form.  mcd_forward and mcd_batch_run run inline unless
- * another form is asked for (the faster form depends on the configuration;
- * mcd/autotune.py measures it).  Setting more than one flag is an error. */
+ * identical in every form.  Without a flag (and without block or
+ * waves_per_simd), mcd_forward runs DS launches of at least 2^21 electrons
+ * served and smaller ones inline; mcd_batch_run runs inline unless deferral is
+ * asked for (the faster form depends on the configuration; mcd/autotune.py
+ * measures it).  Setting more than one flag is an error. */
 #define MCD_LAUNCH_INLINE_GUARD 1
 #define MCD_LAUNCH_DEFER_GUARD 2
 #define MCD_LAUNCH_SERVICE_GUARD 4

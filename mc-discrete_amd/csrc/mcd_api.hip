@@ -475,6 +475,10 @@ extern "C" int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap, i
 // (128 VGPRs, no spills) hides more of the table and LDS latency than the
 // compiler's own allocation (3 per SIMD) at every configuration measured;
 // 6 and 8 spill.  MCD_MINW overrides it.
+// Smallest DS FP32 launch that takes the served guard when the caller names
+// neither a guard placement nor a launch shape.
+static const uint64_t kServedDefaultMin = 1ull << 21;
+
 static int default_minw(const mcd_config&) {
   const char* s = getenv("MCD_MINW");
   if (s) return atoi(s);
@@ -527,15 +531,20 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   // (DS, FP32 only; continuous mode has no guard)
   const bool ds32 = cfg->mode == MCD_MODE_DS && precision == MCD_PREC_FP32;
   const bool deferred = ds32 && (flags & MCD_LAUNCH_DEFER_GUARD);
-  const bool service = ds32 && (flags & MCD_LAUNCH_SERVICE_GUARD);
-  int block = launch && launch->block > 0 ? launch->block : (service ? 448 : 256);
+  // with no placement and no shape asked for, DS launches of at least
+  // kServedDefaultMin electrons take the served guard (on MI355X the fastest
+  // form from 2^21 electrons up; below it the inline form is)
+  const bool shape_given = launch && (launch->block > 0 || launch->waves_per_simd > 0);
+  const bool service = ds32 && ((flags & MCD_LAUNCH_SERVICE_GUARD) ||
+                                (!guard_flags && !shape_given && n >= kServedDefaultMin));
+  int block = launch && launch->block > 0 ? launch->block : (service ? 1024 : 256);
   if (service) {
     if (block % 64 || block < 128 || block > 1024)
       return set_err(ctx, MCD_ERR_INVALID, "service guard: block %d must be a multiple of 64 in [128, 1024]", block);
   } else if (block % 64 || block > 256) {
     return set_err(ctx, MCD_ERR_INVALID, "block %d must be a multiple of 64, at most 256", block);
   }
-  int minw = launch && launch->waves_per_simd > 0 ? launch->waves_per_simd : (service ? 7 : default_minw(*cfg));
+  int minw = launch && launch->waves_per_simd > 0 ? launch->waves_per_simd : (service ? 8 : default_minw(*cfg));
   if (service ? !valid_svc_minw(minw) : !valid_minw(minw))
     return set_err(ctx, MCD_ERR_INVALID, service ? "service guard: waves_per_simd %d not 6, 7 or 8"
                                                  : "waves_per_simd %d not one of 1, 4, 5, 6, 8", minw);

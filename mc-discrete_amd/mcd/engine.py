@@ -120,8 +120,10 @@ class Transport:
     def launch(self, n, els, counters, seed=1, first=0, config_id=0, scat_rows=None, scat_max_electron=0,
                precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None, stream=None):
         """Asynchronous launch on `stream` into caller-owned torch buffers.
-        guard: None (library default: inline), "inline" or "deferred" -- where
-        the DS FP64 guard pick runs (same records either way)."""
+        guard: None (library default: served for DS launches of at least 2^21
+        electrons with no shape given, inline otherwise), "inline", "deferred"
+        or "service" -- where the DS FP64 guard pick runs (same records in
+        every form)."""
         if self.tuned is not None and precision == _abi.PREC_FP32:
             block = block or self.tuned["block"]
             waves_per_simd = waves_per_simd or self.tuned["waves_per_simd"]

@@ -307,6 +307,19 @@ def test_guard_forms_equal_inline_guard(cu20, si10, guard, shape):
     assert ra.scats.tobytes() == rb.scats.tobytes()
 
 
+def test_library_default_from_2_21_electrons_is_served_with_the_same_records(cu20):
+    """With no guard placement and no launch shape, a DS launch of 2^21
+    electrons runs the served form and one of 2^21 - 1 the inline form; both
+    write the inline form's records."""
+    tr = cu20[0]
+    n = 1 << 21
+    a = tr.run(n, seed=2)
+    b = tr.run(n, seed=2, guard="inline")
+    assert a.els.tobytes() == b.els.tobytes()
+    c = tr.run(n - 1, seed=2)
+    assert c.els.tobytes() == b.els[: n - 1].tobytes()
+
+
 @pytest.mark.parametrize("material,E0,Emin", [("Au", 30000.0, 1000.0), ("Al", 5000.0, 500.0)])
 def test_service_guard_other_channel_counts(material, E0, Emin):
     """The served guard at 11 channels (Au, four Gryzinski shells listed plus
