@@ -76,6 +76,42 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* cnt, const
   }
 }
 
+// Counts of records i0, i0 + step, ... < n: each thread reads only the
+// records' outcome dword (byte 32: outcome, outcome2, flags), the wavefronts
+// reduce by shuffles, the block sums its wavefronts in `part` (one row of 5
+// per wavefront, blockDim.x <= 256) and adds once per counter to out[0..4]
+// (electrons, bks, abs, trsm, long).
+static_assert(sizeof(mcd_electron_rec) == 36 && offsetof(mcd_electron_rec, outcome) == 32 &&
+                  offsetof(mcd_electron_rec, outcome2) == 33,
+              "count_outcome_words reads the outcome bytes as dword 8 of a 9-dword record");
+__device__ __forceinline__ void count_outcome_words(const mcd_electron_rec* __restrict__ els, uint64_t n, uint64_t i0,
+                                                    uint64_t step, uint32_t (*part)[5], unsigned long long* out) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(els) + 8;
+  uint32_t v[5] = {0u, 0u, 0u, 0u, 0u};
+  for (uint64_t i = i0; i < n; i += step) {
+    const uint32_t x = __builtin_nontemporal_load(w + 9 * i);
+    const uint32_t o1 = x & 0xFFu, o2 = (x >> 8) & 0xFFu;
+    v[0] += 1u;
+    v[1] += o1 == MCD_OUT_BKS;
+    v[2] += (uint32_t)(o1 == MCD_OUT_ABS) + (uint32_t)(o2 == MCD_OUT_ABS);
+    v[3] += o1 == MCD_OUT_TRSM;
+    v[4] += o1 == MCD_OUT_LONG;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    uint32_t x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (lane == 0) part[wv][k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    unsigned long long t = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += part[q][threadIdx.x];
+    if (t) atomicAdd(out + threadIdx.x, t);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Service guard (DS, FP32, one kernel).  Wavefront 0 of a workgroup is a
 // service wavefront: a transport wavefront posts the guard-band picks of a step
@@ -633,20 +669,13 @@ int batched_blocks_per_cu(int phase, int block, int slice_floats) {
   return nb;
 }
 
-__global__ void k_count_outcomes_batched(const BatchItemDev* __restrict__ items, unsigned long long* counters) {
+__global__ __launch_bounds__(256) void k_count_outcomes_batched(const BatchItemDev* __restrict__ items,
+                                                                unsigned long long* counters) {
+  __shared__ uint32_t part[4][5];
   const BatchItemDev& it = items[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  uint64_t v[5] = {0, 0, 0, 0, 0};
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.R.n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const int o1 = it.R.els[i].outcome, o2 = it.R.els[i].outcome2;
-    v[0] += 1;
-    v[1] += (o1 == MCD_OUT_BKS);
-    v[2] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
-    v[3] += (o1 == MCD_OUT_TRSM);
-    v[4] += (o1 == MCD_OUT_LONG);
-  }
-  wave_add_counters(counters + (size_t)blockIdx.y * MCD_NCOUNTERS + MCD_CNT_ELECTRONS, v, 5, lane);
+  count_outcome_words(it.R.els, it.R.n, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                      (uint64_t)gridDim.x * blockDim.x, part,
+                      counters + (size_t)blockIdx.y * MCD_NCOUNTERS + MCD_CNT_ELECTRONS);
 }
 
 hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
@@ -808,27 +837,19 @@ __global__ __launch_bounds__(256) void k_transport_f64(DevCfg c, RunArgs R) {
 }
 
 // ---------------------------------------------------------------------------
-// Outcome counts from the electron records: electrons, bks, abs (both rows of
-// an exit-then-absorb electron count), trsm, long.
-__global__ void k_count_outcomes(const mcd_electron_rec* __restrict__ els, uint64_t n,
-                                 unsigned long long* __restrict__ counters) {
-  const int lane = threadIdx.x & 63;
-  uint64_t v[5] = {0, 0, 0, 0, 0};
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const int o1 = els[i].outcome, o2 = els[i].outcome2;
-    v[0] += 1;
-    v[1] += (o1 == MCD_OUT_BKS);
-    v[2] += (o1 == MCD_OUT_ABS) + (o2 == MCD_OUT_ABS);
-    v[3] += (o1 == MCD_OUT_TRSM);
-    v[4] += (o1 == MCD_OUT_LONG);
-  }
-  wave_add_counters(counters + MCD_CNT_ELECTRONS, v, 5, lane);
+// Outcome counts from the electron records: electrons, bks, abs (both rows
+// of an exit-then-absorb electron count), trsm, long.
+__global__ __launch_bounds__(256) void k_count_outcomes(const mcd_electron_rec* __restrict__ els, uint64_t n,
+                                                        unsigned long long* __restrict__ counters) {
+  __shared__ uint32_t part[4][5];
+  count_outcome_words(els, n, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x,
+                      part, counters + MCD_CNT_ELECTRONS);
 }
 
 hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsigned long long* counters,
                                  hipStream_t s) {
   uint64_t blocks = (n + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_count_outcomes, dim3((unsigned)blocks), dim3(256), 0, s, els, n, counters);
   return hipGetLastError();
 }
