@@ -44,6 +44,12 @@ class RunResult:
         return int(self.counters[_abi.CNT_EVENTS])
 
 
+class ScatterOverflow(RuntimeError):
+    """The scatter rows of a run exceeded the row buffer (no rows are lost on
+    the device side beyond the capacity being full: rerunning the same
+    electrons with more room, or in smaller ranges, gives the same rows)."""
+
+
 def guard_flags(guard):
     """mcd_launch.flags for a guard placement: None, "inline", "deferred" or
     "service" (the last for single-configuration launches only)."""
@@ -175,7 +181,7 @@ class Transport:
         if rows is not None:
             used = int(counters[_abi.CNT_SCAT_ROWS])
             if used > scat_capacity:
-                raise RuntimeError(f"scatter-row buffer overflow: {used} rows requested, capacity {scat_capacity}")
+                raise ScatterOverflow(f"scatter-row buffer overflow: {used} rows requested, capacity {scat_capacity}")
             srows = rows.cpu().numpy()[: used * _abi.SCATTER_DTYPE.itemsize].view(_abi.SCATTER_DTYPE)
             srows = srows[np.lexsort((srows["step"], srows["electron"]))]
         return RunResult(els=e, counters=counters, scats=srows)

@@ -20,6 +20,8 @@ within a run; the reference instead re-seeded per job with pid*time
 """
 from __future__ import annotations
 
+import numpy as np
+
 from .engine import Transport
 from .fileTools import els_columns, per_electron_dicts, scats_columns, zipDict
 from .material import Material
@@ -101,7 +103,8 @@ def _run(inputPar, numTraj, material, tables, thingsToSave, mode):
     tr = tables.transport(inputPar, material, mode)
     first = tables.claim(numTraj)
     want_scats = bool(thingsToSave.get("scat_output"))
-    r = tr.run(int(numTraj), seed=int(inputPar.get("seed", 0)), first=first, scats=want_scats)
+    seed = int(inputPar.get("seed", 0))
+    r = run_rows(tr, int(numTraj), seed, first) if want_scats else tr.run(int(numTraj), seed=seed, first=first)
     return per_electron_dicts(r.els, r.scats, tr.mat, thingsToSave, first)
 
 
@@ -141,6 +144,33 @@ def scatterMultiEl_cont(inputPar, thingsToSave, output, num, count=0):
     return output
 
 
+# Scatter rows per device buffer when the caller gives no capacity: 2^26 rows
+# of 64 bytes (4 GiB); a range whose rows do not fit is split (run_rows).
+SCAT_ROWS_DEFAULT = 1 << 26
+
+
+def run_rows(tr, n, seed, first, scat_max_electron=None, scat_capacity=None):
+    """Electrons [first, first+n) with their scatter rows, in as many device
+    runs as the row buffer needs: when a range's rows overflow the buffer it is
+    run again as two halves (records and rows are keyed by electron id and the
+    Philox counter, so the halves give exactly the rows of the whole).  Returns
+    one RunResult with the records, rows (sorted by electron, step) and summed
+    counters of the range."""
+    from .engine import RunResult, ScatterOverflow
+    cap = int(scat_capacity) if scat_capacity else min(SCAT_ROWS_DEFAULT, max(1024, 4096 * int(n)))
+    try:
+        return tr.run(int(n), seed=seed, first=first, scats=True, scat_max_electron=scat_max_electron,
+                      scat_capacity=cap)
+    except ScatterOverflow:
+        if n <= 1:
+            raise
+    h = int(n) // 2
+    a = run_rows(tr, h, seed, first, scat_max_electron, cap)
+    b = run_rows(tr, int(n) - h, seed, first + h, scat_max_electron, cap)
+    return RunResult(els=np.concatenate([a.els, b.els]), counters=a.counters + b.counters,
+                     scats=np.concatenate([a.scats, b.scats]))
+
+
 def run_columns(inputPar, n, tables=None, first=None, scat_max_electron=None, scat_capacity=None):
     """Columnar run of n electrons: returns (els_columns, scats_columns or
     None, RunResult).  scats are recorded for electrons with id <
@@ -151,12 +181,15 @@ def run_columns(inputPar, n, tables=None, first=None, scat_max_electron=None, sc
     first = tables.claim(n) if first is None else first
     sc_want = list(inputPar.get("scatter_output", ()))
     el_want = list(inputPar.get("electron_output", ("outcome", "final_E", "final_dir", "last_pos")))
-    r = tr.run(int(n), seed=int(inputPar.get("seed", 0)), first=first, scats=bool(sc_want),
-               scat_max_electron=scat_max_electron, scat_capacity=scat_capacity)
+    seed = int(inputPar.get("seed", 0))
+    if sc_want:
+        r = run_rows(tr, int(n), seed, first, scat_max_electron, scat_capacity)
+    else:
+        r = tr.run(int(n), seed=seed, first=first)
     els = els_columns(r.els, el_want)
     scats = scats_columns(r.scats, tr.mat, sc_want) if sc_want else None
     return els, scats, r
 
 
 __all__ = ["genTables", "multiTraj_DS", "multiTraj_cont", "scatterMultiEl_DS", "scatterMultiEl_cont",
-           "run_columns", "DeviceTables"]
+           "run_columns", "run_rows", "DeviceTables"]

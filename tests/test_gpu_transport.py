@@ -250,9 +250,25 @@ def test_invalid_configuration_is_rejected_before_launch():
 
 
 def test_scatter_buffer_overflow_is_reported(si10):
+    from mcd.engine import ScatterOverflow
     tr = si10[0]
-    with pytest.raises(RuntimeError, match="overflow"):
+    with pytest.raises(ScatterOverflow, match="overflow"):
         tr.run(64, seed=1, scats=True, scat_capacity=100)
+
+
+def test_scatter_rows_split_into_ranges_equal_one_run(si10):
+    """run_rows splits an electron range whose scatter rows overflow the row
+    buffer into halves until each fits; the records, rows and counters equal
+    one run with room for every row."""
+    from mcd.multiScatter import run_rows
+    tr = si10[0]
+    n = 512
+    whole = tr.run(n, seed=0, first=5, scats=True)
+    per_el = len(whole.scats) / n
+    parts = run_rows(tr, n, 0, 5, scat_capacity=int(40 * per_el))  # about 1/13 of the rows per device run
+    assert parts.els.tobytes() == whole.els.tobytes()
+    assert parts.scats.tobytes() == whole.scats.tobytes()
+    assert (parts.counters[:_abi.CNT_SCAT_ROWS] == whole.counters[:_abi.CNT_SCAT_ROWS]).all()
 
 
 def test_reference_statistical_checks_on_gpu_output(si10):
