@@ -166,7 +166,10 @@ class Transport:
             if scat_max_electron is None:
                 scat_max_electron = first + n
             if scat_capacity is None:
-                scat_capacity = max(1024, 4096 * min(n, max(0, scat_max_electron - first)))
+                # room for 4096 rows per recorded electron, at most 2^26 rows (4 GiB);
+                # a run that needs more raises ScatterOverflow (multiScatter.run_rows
+                # splits the range instead)
+                scat_capacity = min(1 << 26, max(1024, 4096 * min(n, max(0, scat_max_electron - first))))
             rows = torch.empty(scat_capacity * _abi.SCATTER_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         self.launch(n, els, cnt, seed=seed, first=first, config_id=config_id, scat_rows=rows,
                     scat_max_electron=scat_max_electron or 0, precision=precision, block=block,
