@@ -4,16 +4,18 @@
 Workload: plain DS mode, semi-infinite Cu (Z=29), E0 = 20 keV tracked to
 Emin = 1 keV, s_tilt = 70 deg, Wc = 50 eV, tol_E = tol_W = 1e-3, 2^22 electrons,
 Philox seed 1.  One step = one mcd_forward launch over the whole electron batch
-(tables built beforehand, outputs resident in HBM).  The channel-pick table and
-the W tables are built once outside the timed region.
+(tables built beforehand, outputs resident in HBM), at the launch shape the
+autotuner picks before the timed region (guard placement, workgroup size,
+waves per SIMD; records are identical for every shape).  The channel-pick
+table and the W tables are built once outside the timed region.
 
 It also reports the 64-configuration batched launch against 64 sequential
 launches (same GPU, same Philox keys).  The second half of the metric, the
 backward/forward time ratio, is measured
 on the differentiation workload: Cu 20 keV -> 1 keV, s_tilt 70 deg, DS
 (Wc 50 eV) and continuous JL, 2^24 electrons, seed 4; forward = one
-mcd_forward launch, backward = one mcd_backward launch (vector-Jacobian
-product for the yield) over the same electrons.
+mcd_forward launch at its autotuned shape, backward = one mcd_backward launch
+(vector-Jacobian product for the yield) over the same electrons.
 
 Prints one JSON line.  With --gpus N > 1 it exits: the benchmark measures one GPU.
 """
