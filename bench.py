@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-backward", action="store_true", help="skip the backward/forward ratio")
     ap.add_argument("--bwd-electrons", type=int, default=1 << 24)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-configuration batched comparison")
+    ap.add_argument("--no-two-stream", action="store_true",
+                    help="skip the two-stream pipelined throughput (reported beside value)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one device-to-device copy of this size (the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE readings)")
@@ -137,6 +139,40 @@ def backward_ratio(n, reps=3):
                     "forward_shape": {k: shape[k] for k in ("block", "waves_per_simd", "guard")}}
     out["ratio_max"] = max(out["DS"]["ratio"], out["cont_JL"]["ratio"])
     return out
+
+
+def two_stream(tr, n, steps, seed):
+    """The serial loop's steps with consecutive launches alternating between
+    two HIP streams (default priority), each with its own record and counter
+    buffers: the next batch starts on the CUs that the previous launch's last
+    trajectories leave idle.  Same launches, same work per step."""
+    import torch
+
+    from mcd import _abi
+    dev = torch.device("cuda", 0)
+    ss = [torch.cuda.Stream(device=dev) for _ in range(2)]
+    bufs = [(torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev),
+             torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)) for _ in range(2)]
+
+    def go(k):
+        els, cnt = bufs[k % 2]
+        with torch.cuda.stream(ss[k % 2]):
+            tr.launch(n, els, cnt, seed=seed, stream=ss[k % 2])
+
+    go(0)
+    go(1)
+    torch.cuda.synchronize(dev)
+    for _, cnt in bufs:
+        cnt.zero_()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        go(k)
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    events = sum(int(cnt[_abi.CNT_EVENTS].item()) for _, cnt in bufs)
+    return {"events_per_s": events / wall, "ms_per_step": wall / steps * 1e3, "streams": 2,
+            "note": "consecutive batches overlap: reported beside value, not as value"}
 
 
 def batched_ratio(reps=3):
@@ -265,6 +301,8 @@ def main():
                           "autotuned": True, "candidates_ms": tuned["ms"]} if tuned is not None else
                          {"block": a.block, "waves_per_simd": a.waves_per_simd, "guard": a.guard, "autotuned": False}),
     }
+    if not a.no_two_stream:
+        out["two_stream"] = two_stream(tr, n, a.steps, a.seed)
     if not a.no_backward:
         bw = backward_ratio(a.bwd_electrons)
         out["backward"] = bw

@@ -13,8 +13,11 @@
  *
  * Device pointers passed in are plain HIP device addresses; streams are
  * hipStream_t passed as void* (NULL = the null stream).  One context is used
- * by one host thread and one stream at a time: calls on one context that
- * overlap in time race on its work-queue counters.
+ * by one host thread at a time.  mcd_forward keeps a launch's work counter in
+ * the caller's counters block (MCD_CNT_QUEUE), so inline or served launches
+ * with separate counters blocks may run concurrently on different streams of
+ * one context; deferred launches and mcd_backward use counters held by the
+ * context, so such calls on one context must not overlap in time.
  */
 #ifndef MCD_H
 #define MCD_H
@@ -176,7 +179,7 @@ enum {
   MCD_CNT_LONG = 5,
   MCD_CNT_FP64_PICKS = 6, /* picks settled by the FP64 guard */
   MCD_CNT_SCAT_ROWS = 7,  /* scatter rows requested (may exceed capacity) */
-  MCD_CNT_QUEUE = 8,      /* internal: persistent-kernel work counter */
+  MCD_CNT_QUEUE = 8,      /* internal: the launch's work counter (mcd_forward resets it) */
   MCD_CNT_GUARD_TIMEOUTS = 9, /* service guard: picks left unsettled (must be 0) */
   MCD_NCOUNTERS = 16
 };

@@ -18,7 +18,7 @@ using namespace mcd;
 struct mcd_ctx {
   int device;
   int n_cu;
-  unsigned long long* d_queue;  // [0] work counter, [1] parked electrons, [2] resume counter
+  unsigned long long* d_queue;  // [0] backward work counter, [1] parked electrons, [2] resume counter
   ParkRec* d_park;              // deferred-guard park buffer
   uint64_t park_cap;            // its capacity in electrons
   char err[512];
@@ -516,7 +516,11 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   R.n = n;
   R.els = els;
   R.counters = (unsigned long long*)counters;
-  R.queue = ctx->d_queue;
+  // the launch's work counter is slot MCD_CNT_QUEUE of the caller's counters
+  // block, so launches with separate blocks share no device state (the
+  // deferred form still uses the context's park buffer and counts)
+  unsigned long long* q = (unsigned long long*)counters + MCD_CNT_QUEUE;
+  R.queue = q;
   if (scats && scats->rows && scats->capacity) {
     R.rows = scats->rows;
     R.row_cap = scats->capacity;
@@ -555,7 +559,7 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)svc_blocks_per_cu(c, minw, block);
     if (grid > want_svc) grid = want_svc;
     if (grid < 1) grid = 1;
-    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
+    HIP_TRY(ctx, launch_fill_u64(q, 0ull, 1, s));
     HIP_TRY(ctx, launch_transport_svc(c, R, minw, (int)grid, block, s));
   } else if (deferred) {
     if (ctx->park_cap < n) {
@@ -576,7 +580,8 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     uint64_t g2 = (uint64_t)ctx->n_cu * (uint64_t)deferred_blocks_per_cu(c, 2, block);
     if (g1 > want) g1 = want;
     if (g2 > want) g2 = want;
-    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 3, s));
+    HIP_TRY(ctx, launch_fill_u64(q, 0ull, 1, s));
+    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue + 1, 0ull, 2, s));
     HIP_TRY(ctx, launch_transport_deferred(c, R, 1, (int)(g1 < 1 ? 1 : g1), block, s));
     HIP_TRY(ctx, launch_transport_deferred(c, R, 2, (int)(g2 < 1 ? 1 : g2), block, s));
   } else {
@@ -584,7 +589,7 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     uint64_t grid = (uint64_t)ctx->n_cu * (uint64_t)per_cu;
     if (grid > want) grid = want;
     if (grid < 1) grid = 1;
-    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue, 0ull, 1, s));
+    HIP_TRY(ctx, launch_fill_u64(q, 0ull, 1, s));
     HIP_TRY(ctx, launch_transport(c, R, precision, minw, (int)grid, block, s));
   }
   HIP_TRY(ctx, launch_count_outcomes(els, n, (unsigned long long*)counters, s));

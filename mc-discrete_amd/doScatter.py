@@ -98,7 +98,7 @@ class Sink:
 
 def main(argv=None):
     inputfile, outdir, chunk = dealWithInput(sys.argv[1:] if argv is None else argv)
-    from mcd.multiScatter import genTables, run_columns
+    from mcd.multiScatter import genTables, iter_columns, run_columns
 
     p = readInput(inputfile)
     p.setdefault("thickness", float("inf"))
@@ -110,6 +110,14 @@ def main(argv=None):
     os.makedirs(outdir, exist_ok=True)
     sink = Sink(os.path.join(outdir, store_name(p)), p)
     done, events = 0, 0
+    if not p.get("scatter_output"):
+        # electron records only: launches pipelined over two streams, the host
+        # writing one chunk while the device runs the next
+        for els, r in iter_columns(p, p["num_el"], chunk, tables=tables, first=0):
+            sink.append("els", els)
+            done += len(r.els)
+            events += r.events
+            print(f"   {done}/{p['num_el']} trajectories, {events} scatter events", flush=True)
     while done < p["num_el"]:
         n = min(chunk, p["num_el"] - done)
         els, scats, r = run_columns(p, n, tables=tables, first=done)

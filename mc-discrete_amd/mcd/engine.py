@@ -191,6 +191,45 @@ class Transport:
 
 
     # -- differentiation ------------------------------------------------------
+    def run_chunks(self, n, chunk, seed=1, first=0, streams=2):
+        """Electrons [first, first+n) in launches of `chunk` electrons, yielded
+        in order as (first electron, RunResult without scatter rows).  The
+        launches alternate between `streams` HIP streams (default priority),
+        each with its own record and counter buffers, so the device starts the
+        next launch while the previous one's last trajectories finish and while
+        the host handles the previous results.  Records equal those of one
+        launch over the whole range (they are keyed by electron id)."""
+        dev = torch.device("cuda", self.device)
+        esz = _abi.ELECTRON_DTYPE.itemsize
+        chunk = max(1, int(chunk))
+        ss = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
+        bufs = [(torch.empty(chunk * esz, dtype=torch.uint8, device=dev),
+                 torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)) for _ in ss]
+        starts = list(range(int(first), int(first) + int(n), chunk))
+
+        def launch(k):
+            s, (els, cnt) = ss[k % len(ss)], bufs[k % len(ss)]
+            m = min(chunk, int(first) + int(n) - starts[k])
+            with torch.cuda.stream(s):
+                cnt.zero_()
+                self.launch(m, els, cnt, seed=seed, first=starts[k], stream=s)
+            return k, m
+
+        pending = [launch(k) for k in range(min(len(ss), len(starts)))]
+        nxt = len(pending)
+        while pending:
+            k, m = pending.pop(0)
+            s, (els, cnt) = ss[k % len(ss)], bufs[k % len(ss)]
+            with torch.cuda.stream(s):  # copies on the finished launch's own stream
+                counters = cnt.cpu().numpy().view(np.uint64).copy()
+                e = els[: m * esz].cpu().numpy().view(_abi.ELECTRON_DTYPE).copy()
+            if nxt < len(starts):
+                pending.append(launch(nxt))
+                nxt += 1
+            if counters[_abi.CNT_GUARD_TIMEOUTS]:
+                raise RuntimeError(f"{int(counters[_abi.CNT_GUARD_TIMEOUTS])} guard picks were not settled")
+            yield starts[k], RunResult(els=e, counters=counters, scats=None)
+
     def _bwd(self, n, seed, first, config_id, grad_out, out_vjp, out_jac, n_bins, rel_step, graze, block,
              stream):
         ctx = context(self.device)

@@ -171,6 +171,20 @@ def run_rows(tr, n, seed, first, scat_max_electron=None, scat_capacity=None):
                      scats=np.concatenate([a.scats, b.scats]))
 
 
+def iter_columns(inputPar, n, chunk, tables=None, first=None, streams=2):
+    """Columnar runs of n electrons in launches of `chunk`, pipelined over
+    `streams` HIP streams (Transport.run_chunks); yields (els_columns,
+    RunResult) per launch in electron order.  Scatter rows are not recorded
+    here (run_columns records them)."""
+    _check_elastic(inputPar)
+    tables = _tables_for(inputPar, tables)
+    tr = tables.transport(inputPar, None, inputPar.get("mode", "DS"))
+    first = tables.claim(n) if first is None else first
+    el_want = list(inputPar.get("electron_output", ("outcome", "final_E", "final_dir", "last_pos")))
+    for _, r in tr.run_chunks(int(n), int(chunk), seed=int(inputPar.get("seed", 0)), first=first, streams=streams):
+        yield els_columns(r.els, el_want), r
+
+
 def run_columns(inputPar, n, tables=None, first=None, scat_max_electron=None, scat_capacity=None):
     """Columnar run of n electrons: returns (els_columns, scats_columns or
     None, RunResult).  scats are recorded for electrons with id <
@@ -192,4 +206,4 @@ def run_columns(inputPar, n, tables=None, first=None, scat_max_electron=None, sc
 
 
 __all__ = ["genTables", "multiTraj_DS", "multiTraj_cont", "scatterMultiEl_DS", "scatterMultiEl_cont",
-           "run_columns", "run_rows", "DeviceTables"]
+           "run_columns", "iter_columns", "run_rows", "DeviceTables"]

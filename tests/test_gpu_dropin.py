@@ -77,3 +77,28 @@ def test_doscatter_cli_writes_tables(tmp_path):
     assert list(scats.columns) == ["type", "E", "energy"]
     assert np.all(scats["E"] >= scats["energy"])
     assert set(scats["type"]) <= {"Ruth", "Moller", "Quinn", "Gryz1s", "Gryz2s", "Gryz2p"}
+
+
+def test_doscatter_without_scatter_rows_pipelines_launches(tmp_path):
+    """With no scatter_output, doScatter runs its chunks as launches pipelined
+    over two streams; the els table equals one launch over all electrons."""
+    import pandas as pd
+    import doScatter
+    from mcd.multiScatter import genTables, run_columns
+    f = tmp_path / "in.file"
+    text = ("num_el : 2500\nE0 : 10000\nEmin : 1000\nmaterial : Si\ns_tilt : 70\nelastic : Ruth_vanilla\n"
+            "mode : DS\nBethe_model : classical\ndiffMFP : no\ngen_tables : yes\ntol_E : 1e-2\ntol_W : 1e-2\n"
+            "maxScatt : off\nWc : 100\nseed : 0\nelectron_output : outcome, final_E, final_dir, last_pos\n")
+    f.write_text(text)
+    path = doScatter.main(["-i", str(f), "-o", str(tmp_path / "out"), "-c", "1000"])
+    if path.endswith(".h5") and os.path.isfile(path):
+        els = pd.read_hdf(path, "els")
+    else:
+        els = pd.concat([pd.read_parquet(os.path.join(path, f)) for f in sorted(os.listdir(path))
+                         if f.startswith("els")], ignore_index=True)
+    from mcd.fileTools import readInput
+    p = readInput(str(f))
+    one, _, _ = run_columns(p, 2500, tables=genTables(p), first=0)
+    assert len(els) == len(one["final_E"])
+    assert np.array_equal(els["final_E"].to_numpy(), np.asarray(one["final_E"]))
+    assert list(els["outcome"]) == list(one["outcome"])

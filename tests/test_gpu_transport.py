@@ -256,6 +256,22 @@ def test_scatter_buffer_overflow_is_reported(si10):
         tr.run(64, seed=1, scats=True, scat_capacity=100)
 
 
+def test_pipelined_chunks_on_two_streams_equal_one_launch(cu20):
+    """Launches alternating between two streams, each with its own counters
+    block (which holds its work counter), run concurrently on the device and
+    give the records of one launch over the whole range."""
+    tr = cu20[0]
+    n = (1 << 18) + 5
+    whole = tr.run(n, seed=3, first=11)
+    got, events = [], 0
+    for start, r in tr.run_chunks(n, 1 << 16, seed=3, first=11, streams=2):
+        assert start == 11 + sum(len(x) for x in got)
+        got.append(r.els)
+        events += int(r.counters[_abi.CNT_EVENTS])
+    assert np.concatenate(got).tobytes() == whole.els.tobytes()
+    assert events == int(whole.counters[_abi.CNT_EVENTS])
+
+
 def test_scatter_rows_split_into_ranges_equal_one_run(si10):
     """run_rows splits an electron range whose scatter rows overflow the row
     buffer into halves until each fits; the records, rows and counters equal
