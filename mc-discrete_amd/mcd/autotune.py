@@ -180,7 +180,8 @@ def tune_batch(items, seed=3, candidates=CANDIDATES, reps=2, device=0, use_cache
 
     Returns {"block", "pick_nodes", "guard", "ms": {candidate: ms}, "cached": bool}."""
     path = cache_path or _cache_path()
-    key = batch_key(items)
+    # the key names the candidate set too, so a cached choice is one of today's
+    key = batch_key(items) + hashlib.sha1(",".join(_name(c) for c in candidates).encode()).hexdigest()[:8]
     if use_cache:
         hit = _load(path).get(key)
         if hit and any(_name(hit) == _name(c) for c in candidates):
