@@ -416,3 +416,23 @@ def test_transport_autotuner_picks_a_shape_and_keeps_records(cu20, tmp_path):
         assert tr.run(n, seed=1).els.tobytes() == ref.els.tobytes()
     finally:
         tr.tuned = None
+
+
+def test_served_guard_matches_inline_across_the_batch_configurations():
+    """The served-guard kernel writes the inline form's records for each of
+    the 32 DS configurations of the batched workload ({Al, Si, Cu, Au} x E0
+    {5, 10, 20, 30} keV x tilt {0, 70} deg: 6, 7 and 11 channels), 2^14
+    electrons each, and settles the same number of FP64 guard picks."""
+    from mcd import engine, workloads
+    picks = 0
+    for c in workloads.batched_configs():
+        if c["mode"] != "DS":
+            continue
+        tr = engine.Transport(c["material"], c["E0"], c["Emin"], Wc=c["Wc"], s_tilt=c["s_tilt"], tol_E=c["tol"],
+                              tol_W=c["tol"])
+        a = tr.run(1 << 14, seed=9, guard="service")
+        b = tr.run(1 << 14, seed=9, guard="inline")
+        assert a.els.tobytes() == b.els.tobytes(), c
+        assert int(a.counters[_abi.CNT_FP64_PICKS]) == int(b.counters[_abi.CNT_FP64_PICKS]), c
+        picks += int(a.counters[_abi.CNT_FP64_PICKS])
+    assert picks > 0
