@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-configuration batched comparison")
     ap.add_argument("--no-two-stream", action="store_true",
                     help="skip the two-stream pipelined throughput (reported beside value)")
+    ap.add_argument("--config-sweep", action="store_true",
+                    help="instead of the headline line, print one JSON line per parity configuration "
+                         "(FP64 Si 10 keV, DS Cu 20 keV, continuous Au 30 keV x 3 Bethe laws x 2 tilts)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one device-to-device copy of this size (the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE readings)")
@@ -217,12 +220,57 @@ def batched_ratio(reps=3):
                       "candidates_ms": tuned["ms"]}}
 
 
+def config_sweep(steps):
+    """Throughput of the configurations the parity tests run: the all-FP64
+    kernel on Si 10 keV (2,000 electrons, seed 0), the DS headline (Cu 20 keV,
+    70 deg, 2^22 electrons, seed 1) and continuous Au 30 keV -> 1 keV with the
+    classical, JL and explicit Bethe laws at 0 and 70 deg (2^22 electrons,
+    seed 2), each at its autotuned launch shape (FP64: library default)."""
+    import torch
+
+    from mcd import _abi, engine
+    dev = torch.device("cuda", 0)
+    runs = [("FP64 DS Si 10 keV -> 1 keV, 70 deg, Wc 100 eV, tol 1e-2", 2000, 0,
+             dict(material="Si", E0=10000.0, Emin=1000.0, Wc=100.0, s_tilt=70.0, tol_E=1e-2, tol_W=1e-2), True),
+            ("DS Cu 20 keV -> 1 keV, 70 deg, Wc 50 eV, tol 1e-3", 1 << 22, 1,
+             dict(material="Cu", E0=20000.0, Emin=1000.0, Wc=50.0, s_tilt=70.0), False)]
+    for law in ("classical", "JL", "explicit"):
+        for tilt in (0.0, 70.0):
+            runs.append((f"continuous {law} Au 30 keV -> 1 keV, {tilt:g} deg", 1 << 22, 2,
+                         dict(material="Au", E0=30000.0, Emin=1000.0, s_tilt=tilt, mode="cont", bethe_model=law),
+                         False))
+    for name, n, seed, kw, fp64 in runs:
+        tr = engine.Transport(**kw)
+        prec = _abi.PREC_FP64 if fp64 else _abi.PREC_FP32
+        if not fp64:
+            tr.tune(n, seed=seed, reps=1, use_cache=False)
+        els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+        tr.launch(n, els, cnt, seed=seed, precision=prec)
+        torch.cuda.synchronize(dev)
+        cnt.zero_()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * steps)]
+        for k in range(steps):
+            ev[2 * k].record()
+            tr.launch(n, els, cnt, seed=seed, precision=prec)
+            ev[2 * k + 1].record()
+        torch.cuda.synchronize(dev)
+        ms = min(ev[2 * k].elapsed_time(ev[2 * k + 1]) for k in range(steps))
+        events = int(cnt[_abi.CNT_EVENTS].item()) / steps
+        print(json.dumps({"workload": name, "electrons": n, "seed": seed, "precision": "fp64" if fp64 else "fp32",
+                          "launch_shape": tr.tuned, "ms_per_launch": ms, "events_per_launch": events,
+                          "events_per_s": events / (ms * 1e-3)}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU (asked for {a.gpus}); electron ranges would shard across GPUs, "
               "but no multi-GPU run is launched", file=sys.stderr)
         sys.exit(2)
+    if a.config_sweep:
+        config_sweep(a.steps)
+        return
     import torch
 
     from mcd import _abi, engine
