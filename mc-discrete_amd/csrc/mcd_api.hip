@@ -773,6 +773,8 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
   const int flags = launch ? launch->flags : 0;
   if ((flags & MCD_LAUNCH_INLINE_GUARD) && (flags & MCD_LAUNCH_DEFER_GUARD))
     return set_err(ctx, MCD_ERR_INVALID, "MCD_LAUNCH_INLINE_GUARD and MCD_LAUNCH_DEFER_GUARD both set");
+  if (flags & MCD_LAUNCH_SERVICE_GUARD)
+    return set_err(ctx, MCD_ERR_UNSUPPORTED, "the served guard is built for mcd_forward; batches run inline or deferred");
   // DS items defer the FP64 guard pick to a second phase only when asked to
   const bool deferred = b->has_ds && (flags & MCD_LAUNCH_DEFER_GUARD);
   hipStream_t s = (hipStream_t)stream;

@@ -41,6 +41,17 @@ def test_batched_equals_per_configuration_launches(small_batch, use_graph, guard
             assert int(r.counters[_abi.CNT_ELECTRONS]) == c["n"]
 
 
+def test_batch_rejects_the_served_guard(small_batch):
+    """The served guard is built for single-configuration launches; a batch
+    asked for it reports MCD_ERR_UNSUPPORTED instead of launching."""
+    from mcd import engine
+    from mcd._lib import MCDError
+    cfgs, trs = small_batch
+    b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs[:2], cfgs[:2])], pick_nodes=128)
+    with pytest.raises(MCDError, match="served guard"):
+        b.launch(seed=3, use_graph=False, guard="service")
+
+
 def test_autotuner_picks_and_caches_a_launch_shape(small_batch, tmp_path):
     """The tuner times every candidate, returns the fastest, caches it by the
     batch's configurations, and the tuned batch reproduces per-configuration
