@@ -238,6 +238,8 @@ def test_max_scatt_and_thin_film_outcomes():
         assert (o["outcome"] == out).sum() > 50
     same = identical(r.els, o)
     assert same.mean() >= 0.999
+    for guard in ("service", "deferred"):  # the 'long' and 'trsm' paths in the other guard forms
+        assert tr.run(n, seed=6, guard=guard).els.tobytes() == r.els.tobytes(), guard
 
 
 def test_invalid_configuration_is_rejected_before_launch():
