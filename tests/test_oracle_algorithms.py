@@ -107,9 +107,9 @@ def test_moller_cdf_against_analytic(si10):
         assert np.array_equal(row > 0, ~(t.Ws[1:] > wmax))
 
 
-def test_table_grid_shapes_match_survey_restatement():
+def test_table_grid_shapes_match_independent_restatement():
     """Grid sizes of the tolerance search (probTables.py:280-382) equal the
-    survey's independently restated values: Si 10 keV tol 1e-2 gives 300x29
+    independently restated values: Si 10 keV tol 1e-2 gives 300x29
     (Moller) and 400x39 (Gryz 2p); Cu 20 keV tol 1e-3 gives 2000x199 twice."""
     from conftest import CU20, SI10, build_grids
     _, gm, gg = build_grids(SI10)

@@ -157,7 +157,7 @@ def test_free_electron_constants():
     cu1 = fermi_energy(at_num_dens(8.96, 63.546), 1)
     assert cu1 == pytest.approx(7.00, rel=0.01)
     assert plasmon_energy(at_num_dens(8.96, 63.546), 1) == pytest.approx(10.8, rel=0.02)
-    # survey values (scipy CODATA)
+    # values restated independently from the reference's formulas with scipy's CODATA constants
     cu = Material("Cu")
     assert cu.atnd == pytest.approx(8.4912e28, rel=1e-4)
     assert cu.fermi_e == pytest.approx(34.843, rel=1e-4)
