@@ -139,7 +139,7 @@ def backward_ratio(n, reps=3):
             tb.append(e[2].elapsed_time(e[3]))
         f, b = min(tf), min(tb)
         out[key] = {"forward_ms": f, "backward_ms": b, "ratio": b / f,
-                    "forward_shape": {k: shape[k] for k in ("block", "waves_per_simd", "guard")}}
+                    "forward_shape": {k: shape[k] for k in ("block", "waves_per_simd", "guard", "refill_period")}}
     out["ratio_max"] = max(out["DS"]["ratio"], out["cont_JL"]["ratio"])
     return out
 
@@ -346,6 +346,7 @@ def main():
                      "frac_vs_nonpacked_78_6": achieved / 78.6},
         "cpu_baseline": None,
         "launch_shape": ({"block": tuned["block"], "waves_per_simd": tuned["waves_per_simd"], "guard": tuned["guard"],
+                          "refill_period": tuned.get("refill_period", 1),
                           "autotuned": True, "candidates_ms": tuned["ms"]} if tuned is not None else
                          {"block": a.block, "waves_per_simd": a.waves_per_simd, "guard": a.guard, "autotuned": False}),
     }

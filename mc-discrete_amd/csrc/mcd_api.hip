@@ -528,6 +528,7 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     R.row_cursor = (unsigned long long*)&counters[MCD_CNT_SCAT_ROWS];
   }
   const int flags = launch ? launch->flags : 0;
+  R.refill_mask = (1u << ((flags & MCD_LAUNCH_REFILL_MASK) >> MCD_LAUNCH_REFILL_SHIFT)) - 1u;
   const int guard_flags = flags & (MCD_LAUNCH_INLINE_GUARD | MCD_LAUNCH_DEFER_GUARD | MCD_LAUNCH_SERVICE_GUARD);
   if (guard_flags & (guard_flags - 1))
     return set_err(ctx, MCD_ERR_INVALID, "at most one of MCD_LAUNCH_INLINE_GUARD, _DEFER_GUARD, _SERVICE_GUARD");

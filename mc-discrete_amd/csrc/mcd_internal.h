@@ -29,6 +29,7 @@ struct RunArgs {
   ParkRec* park;                     // deferred guard: parked electrons (capacity n)
   unsigned long long* park_count;    // electrons parked by the first kernel
   unsigned long long* resume_queue;  // work counter of the resume kernel
+  uint32_t refill_mask;              // idle lanes are refilled every (refill_mask + 1) steps
 };
 
 // One configuration of a batched launch (device-resident).

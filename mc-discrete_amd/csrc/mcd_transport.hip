@@ -234,12 +234,15 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
   F2 E = {0.f, 0.f};
   float p[3] = {0.f, 0.f, 0.f}, pc[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
 
+  uint32_t iter = 0;
   while (true) {
     // the key words stay loop-carried SGPR values (set where all lanes are
     // converged): otherwise the compiler hoists all twenty round keys out of
     // the loop and spills them to VGPR lanes, paying a v_readlane per round
     asm volatile("" : "+s"(k0), "+s"(k1));
-    if (open) {
+    // compaction period: idle lanes take new electrons every
+    // (refill_mask + 1) steps, and at once when every lane is idle
+    if (open && ((iter++ & R.refill_mask) == 0u || !__any(active))) {
       uint64_t idle = __ballot(!active);
       if (idle) {
         const uint64_t idx = refill_index(wq, PHASE == 2 ? R.resume_queue : R.queue, idle, lane, n_work, open);

@@ -72,6 +72,8 @@ class Launch(C.Structure):
 LAUNCH_INLINE_GUARD = 1
 LAUNCH_DEFER_GUARD = 2
 LAUNCH_SERVICE_GUARD = 4
+LAUNCH_REFILL_SHIFT = 8
+LAUNCH_REFILL_MASK = 7 << LAUNCH_REFILL_SHIFT
 
 
 class BwdOpts(C.Structure):

@@ -3,11 +3,13 @@ batched runs (mcd_batch_*).
 
 Single configurations (``tune_transport``): ``block`` (threads per workgroup),
 ``waves_per_simd`` (the register budget: the resident wavefronts per SIMD it
-must allow) and ``guard``, where a DS step whose channel pick falls in the
+must allow), ``guard``, where a DS step whose channel pick falls in the
 guard band gets its FP64 pick: inline in the transport code, deferred to a
 second kernel whose first phase carries no FP64 code, or served by wavefront 0
 of each workgroup through LDS (one kernel whose transport code carries no
-FP64).  Records are identical for every shape.
+FP64), and ``refill_period``, the compaction period: a wavefront refills its
+idle lanes with new electrons every that many steps (fewer refill passes,
+lanes idle a little longer).  Records are identical for every shape.
 
 Batched runs (``tune_batch``):
 
@@ -22,8 +24,7 @@ The batched kernel has three shape choices that trade against each other:
   first phase without FP64 code parks those electrons, a second resumes them)
   or ``"inline"`` (one phase, more registers, fewer resident wavefronts).
 
-Electrons are refilled into idle lanes every step (ballot + prefix count), so
-there is no separate compaction period to choose.
+The batched kernel refills idle lanes every step (ballot + prefix count).
 
 ``tune_batch`` times every candidate on the actual batch (graph replay, best
 of ``reps``), keeps the fastest and caches the choice by a key of the
@@ -55,13 +56,16 @@ def _name(c):
 
 
 TRANSPORT_CANDIDATES = (
-    dict(block=256, waves_per_simd=4, guard="inline"),
-    dict(block=128, waves_per_simd=4, guard="inline"),
-    dict(block=256, waves_per_simd=5, guard="inline"),
-    dict(block=256, waves_per_simd=0, guard="deferred"),
-    dict(block=512, waves_per_simd=8, guard="service"),
-    dict(block=1024, waves_per_simd=8, guard="service"),
-    dict(block=512, waves_per_simd=7, guard="service"),
+    dict(block=256, waves_per_simd=4, guard="inline", refill_period=1),
+    dict(block=128, waves_per_simd=4, guard="inline", refill_period=1),
+    dict(block=256, waves_per_simd=5, guard="inline", refill_period=1),
+    dict(block=256, waves_per_simd=4, guard="inline", refill_period=2),
+    dict(block=256, waves_per_simd=0, guard="deferred", refill_period=1),
+    dict(block=512, waves_per_simd=8, guard="service", refill_period=1),
+    dict(block=1024, waves_per_simd=8, guard="service", refill_period=1),
+    dict(block=512, waves_per_simd=7, guard="service", refill_period=1),
+    dict(block=1024, waves_per_simd=8, guard="service", refill_period=2),
+    dict(block=1024, waves_per_simd=8, guard="service", refill_period=4),
 )
 
 
@@ -86,7 +90,8 @@ def transport_key(tr, n) -> str:
 
 
 def _tname(c):
-    return f"block={c['block']},waves_per_simd={c['waves_per_simd']},guard={c['guard']}"
+    return (f"block={c['block']},waves_per_simd={c['waves_per_simd']},guard={c['guard']},"
+            f"refill_period={c.get('refill_period', 1)}")
 
 
 def _time_transport(tr, n, cand, seed, reps, device):
@@ -96,7 +101,8 @@ def _time_transport(tr, n, cand, seed, reps, device):
     dev = torch.device("cuda", device)
     els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
-    kw = dict(block=cand["block"], waves_per_simd=cand["waves_per_simd"], guard=cand["guard"])
+    kw = dict(block=cand["block"], waves_per_simd=cand["waves_per_simd"], guard=cand["guard"],
+              refill_period=cand.get("refill_period", 1))
     saved, tr.tuned = tr.tuned, None
     try:
         tr.launch(n, els, cnt, seed=seed, **kw)
@@ -141,7 +147,8 @@ def tune_transport(tr, n=1 << 22, seed=1, candidates=TRANSPORT_CANDIDATES, reps=
             return dict(hit, cached=True)
     timings = {_tname(c): _time_transport(tr, n, c, seed, reps, device) for c in candidates}
     best = min(candidates, key=lambda c: timings[_tname(c)])
-    out = dict(block=best["block"], waves_per_simd=best["waves_per_simd"], guard=best["guard"], ms=timings)
+    out = dict(block=best["block"], waves_per_simd=best["waves_per_simd"], guard=best["guard"],
+               refill_period=best.get("refill_period", 1), ms=timings)
     if use_cache:
         _store(path, key, out)
     return dict(out, cached=False)

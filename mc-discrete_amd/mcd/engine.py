@@ -64,6 +64,17 @@ def guard_flags(guard):
     raise ValueError(f"guard must be None, 'inline', 'deferred' or 'service', not {guard!r}")
 
 
+def refill_flags(period):
+    """mcd_launch.flags bits for a compaction period of 1, 2, 4, ... 128 steps
+    (0 or None: every step)."""
+    if not period or period == 1:
+        return 0
+    k = int(period).bit_length() - 1
+    if period != 1 << k or k > 7:
+        raise ValueError(f"refill_period must be a power of two up to 128, not {period!r}")
+    return k << _abi.LAUNCH_REFILL_SHIFT
+
+
 class Transport:
     """One configuration on one GPU."""
 
@@ -124,23 +135,25 @@ class Transport:
 
     # -- runs -----------------------------------------------------------------
     def launch(self, n, els, counters, seed=1, first=0, config_id=0, scat_rows=None, scat_max_electron=0,
-               precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None, stream=None):
+               precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None, stream=None, refill_period=0):
         """Asynchronous launch on `stream` into caller-owned torch buffers.
         guard: None (library default: served for DS launches of at least 2^21
         electrons with no shape given, inline otherwise), "inline", "deferred"
         or "service" -- where the DS FP64 guard pick runs (same records in
-        every form)."""
+        every form).  refill_period: the compaction period, idle lanes take
+        new electrons every refill_period steps (0: every step)."""
         if self.tuned is not None and precision == _abi.PREC_FP32:
             block = block or self.tuned["block"]
             waves_per_simd = waves_per_simd or self.tuned["waves_per_simd"]
             guard = guard or self.tuned["guard"]
+            refill_period = refill_period or self.tuned.get("refill_period", 0)
         ctx = context(self.device)
         sb = None
         if scat_rows is not None:
             sb = _abi.ScatterBuf(scat_rows.data_ptr(), scat_rows.numel() // _abi.SCATTER_DTYPE.itemsize,
                                  int(scat_max_electron))
         la = _abi.Launch(int(block), int(waves_per_simd), int(precision),
-                         guard_flags(guard))
+                         guard_flags(guard) | refill_flags(refill_period))
         s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
         check(lib().mcd_forward(ctx, C.byref(self.cfg), self._tables, int(seed), int(config_id), int(first), int(n),
                                 els.data_ptr(), C.byref(sb) if sb is not None else None, counters.data_ptr(),
@@ -152,11 +165,12 @@ class Transport:
         (records are the same for every shape) and return the tuner's report."""
         from .autotune import tune_transport
         res = tune_transport(self, n, **kw)
-        self.tuned = dict(block=res["block"], waves_per_simd=res["waves_per_simd"], guard=res["guard"])
+        self.tuned = dict(block=res["block"], waves_per_simd=res["waves_per_simd"], guard=res["guard"],
+                          refill_period=res.get("refill_period", 1))
         return res
 
     def run(self, n, seed=1, first=0, config_id=0, scats=False, scat_max_electron=None, scat_capacity=None,
-            precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None) -> RunResult:
+            precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None, refill_period=0) -> RunResult:
         """Run electrons [first, first+n) and return host results."""
         dev = torch.device("cuda", self.device)
         els = torch.empty(max(n, 1) * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
@@ -173,7 +187,7 @@ class Transport:
             rows = torch.empty(scat_capacity * _abi.SCATTER_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         self.launch(n, els, cnt, seed=seed, first=first, config_id=config_id, scat_rows=rows,
                     scat_max_electron=scat_max_electron or 0, precision=precision, block=block,
-                    waves_per_simd=waves_per_simd, guard=guard)
+                    waves_per_simd=waves_per_simd, guard=guard, refill_period=refill_period)
         torch.cuda.synchronize(dev)
         counters = cnt.cpu().numpy().view(np.uint64)
         if counters[_abi.CNT_GUARD_TIMEOUTS]:

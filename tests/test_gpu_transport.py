@@ -222,6 +222,8 @@ def test_every_electron_runs_once_at_any_count(cu20, n):
         big = tr.run(n + 40, seed=7, first=3, guard=guard)
         assert big.els[:n].tobytes() == r.els.tobytes(), guard
         assert int(r.counters[_abi.CNT_EVENTS]) == int(r.els["n_scatt"].astype(np.int64).sum()), guard
+        # compaction period: lanes refilled every 8 steps give the same records
+        assert tr.run(n, seed=7, first=3, guard=guard, refill_period=8).els.tobytes() == r.els.tobytes(), guard
 
 
 def test_max_scatt_and_thin_film_outcomes():

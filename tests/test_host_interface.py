@@ -83,9 +83,20 @@ def test_python_constants_match_the_header():
             "MCD_LAUNCH_SERVICE_GUARD": _abi.LAUNCH_SERVICE_GUARD, "MCD_FLAG_FP64_PICK": _abi.FLAG_FP64_PICK,
             "MCD_FLAG_GUARD_TIMEOUT": _abi.FLAG_GUARD_TIMEOUT, "MCD_CNT_EVENTS": _abi.CNT_EVENTS,
             "MCD_CNT_FP64_PICKS": _abi.CNT_FP64_PICKS, "MCD_CNT_QUEUE": _abi.CNT_QUEUE,
-            "MCD_CNT_GUARD_TIMEOUTS": _abi.CNT_GUARD_TIMEOUTS}
+            "MCD_CNT_GUARD_TIMEOUTS": _abi.CNT_GUARD_TIMEOUTS, "MCD_LAUNCH_REFILL_SHIFT": _abi.LAUNCH_REFILL_SHIFT}
     for k, v in want.items():
         assert vals[k] == v, k
+
+
+def test_refill_period_flags():
+    from mcd import _abi
+    from mcd.engine import refill_flags
+    assert refill_flags(0) == 0 and refill_flags(1) == 0
+    assert refill_flags(4) == 2 << _abi.LAUNCH_REFILL_SHIFT
+    assert refill_flags(128) & ~_abi.LAUNCH_REFILL_MASK == 0
+    for bad in (3, 256):
+        with pytest.raises(ValueError):
+            refill_flags(bad)
 
 
 def test_unsupported_reference_options_raise():
