@@ -132,6 +132,37 @@ def test_step_rows_match_oracle_trace(si10):
     assert np.array_equal(s["outcome"], tro["outcome"].astype(np.uint8))
 
 
+def test_step_rows_match_committed_golden_trace():
+    """The device, given the committed golden float32 tables of the
+    CPU-runnable DS case (tests/golden/c1_si10_seed0.npz), reproduces the
+    committed per-step trace and records of its electrons 0..3 (seed 0): same
+    grids, channels, W values, outcomes and step counts; energies, angles,
+    paths and directions to FP32 accuracy."""
+    import os
+    from conftest import SI10
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_si10_seed0.npz"))
+    tr = transport(SI10, cdfs=(g["cdf_m"], g["cdf_g"]))
+    gm, gg = tr.grids
+    for a, b in ((gm.Es, g["Es_m"]), (gm.Ws, g["Ws_m"]), (gg.Es, g["Es_g"]), (gg.Ws, g["Ws_g"])):
+        assert np.array_equal(a, b)
+    assert np.array_equal(tr.cdf(0), g["cdf_m"]) and np.array_equal(tr.cdf(1), g["cdf_g"])
+    n = len(g["els_final_E"])
+    r = tr.run(n, seed=0, scats=True)
+    s = r.scats
+    assert len(s) == len(g["trace_step"])
+    assert np.array_equal(s["electron"], g["trace_electron"]) and np.array_equal(s["step"], g["trace_step"])
+    assert np.array_equal(s["channel"], g["trace_channel"])
+    assert np.array_equal(s["outcome"], g["trace_outcome"].astype(np.uint8))
+    assert np.array_equal(s["E_loss"], g["trace_E_loss"].astype(np.float32))
+    for k, rtol in (("E", 1e-6), ("energy", 1e-6), ("az", 1e-6), ("pathl", 2e-5)):
+        assert np.allclose(s[k], g["trace_" + k], rtol=rtol), k
+    assert np.allclose(s["pol"], g["trace_pol"], rtol=1e-5, atol=1e-6)
+    for k in ("dx", "dy", "dz"):
+        assert np.abs(s[k] - g["trace_" + k]).max() < 2e-5, k
+    assert np.array_equal(r.els["outcome"], g["els_outcome"]) and np.array_equal(r.els["n_scatt"], g["els_n_scatt"])
+    assert np.allclose(r.els["final_E"], g["els_final_E"], rtol=1e-6)
+
+
 def test_headline_config_first_1e4_electrons(cu20):
     """Cu 20 keV, 70 deg, Wc 50, tol 1e-3: the first 10^4 electrons of the
     headline workload trajectory by trajectory."""

@@ -11,6 +11,7 @@
   Watson U^2 polar angles) on oracle output.
 """
 import bisect
+import os
 import math
 import operator
 
@@ -226,3 +227,26 @@ def test_cfg_struct_layout_matches_header(tmp_path):
             C.sizeof(_abi.ScatterBuf), C.sizeof(_abi.Launch), C.sizeof(_abi.BatchItem), C.sizeof(_abi.BwdOpts),
             _abi.BwdOpts.graze.offset, _abi.NGRAD, _abi.GRAD_LNRHO, _abi.MAX_BINS]
     assert got == want
+
+
+def _golden_module():
+    import importlib.util
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_c1_golden.py")
+    spec = importlib.util.spec_from_file_location("make_c1_golden", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_oracle_and_table_restatement_reproduce_the_committed_golden_vectors():
+    """tests/golden/c1_si10_seed0.npz freezes the grids, the float32 CDF tables
+    and the per-step trace of 4 electrons of the CPU-runnable DS case; the
+    grid search, the table restatement and the oracle reproduce every array
+    bit for bit."""
+    mod = _golden_module()
+    ref = np.load(os.path.join(os.path.dirname(mod.__file__), "c1_si10_seed0.npz"))
+    now = mod.golden()
+    assert set(ref.files) == set(now)
+    for k in ref.files:
+        assert np.array_equal(ref[k], now[k]), k
+    assert len(ref["trace_step"]) > 1000 and set(ref["els_outcome"]) <= {1, 2}
