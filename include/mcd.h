@@ -198,14 +198,15 @@ typedef struct mcd_launch {
 /* DS in FP32 can run the FP64 guard pick inline (one kernel), deferred (two
  * kernels: the first has no FP64 code and parks an electron whose channel pick
  * falls in the guard band, the second resumes the parked electrons with the
- * FP64 pick inline) or served (mcd_forward only: one kernel in which wavefront
- * 0 of each workgroup settles the guard-band picks that the other wavefronts
- * post to it through LDS, so the transport code carries no FP64).  Records are
- * identical in every form.  Without a flag (and without block or
- * waves_per_simd), mcd_forward runs DS launches of at least 2^21 electrons
- * served and smaller ones inline; mcd_batch_run runs inline unless deferral is
- * asked for (the faster form depends on the configuration; mcd/autotune.py
- * measures it).  Setting more than one flag is an error. */
+ * FP64 pick inline) or served (one kernel in which wavefront 0 of each
+ * workgroup settles the guard-band picks that the other wavefronts post to it
+ * through LDS, so the transport code carries no FP64; for mcd_batch_run the
+ * block may then be 128 to 1024).  Records are identical in every form.
+ * Without a flag (and without block or waves_per_simd), mcd_forward runs DS
+ * launches of at least 2^21 electrons served and smaller ones inline;
+ * mcd_batch_run runs inline unless another form is asked for (the faster form
+ * depends on the configuration; mcd/autotune.py measures it).  Setting more
+ * than one flag is an error. */
 #define MCD_LAUNCH_INLINE_GUARD 1
 #define MCD_LAUNCH_DEFER_GUARD 2
 #define MCD_LAUNCH_SERVICE_GUARD 4

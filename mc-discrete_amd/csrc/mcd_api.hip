@@ -613,7 +613,7 @@ struct mcd_batch {
   uint64_t graph_seed;
   uint64_t* graph_counters;
   hipStream_t graph_stream;
-  int graph_block, graph_grid, graph_deferred;
+  int graph_block, graph_grid, graph_deferred;  // graph_deferred: the guard form the graph was captured with
 };
 
 static double work_per_electron(const mcd_config& c) {
@@ -746,11 +746,16 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
   return MCD_OK;
 }
 
-static hipError_t batch_enqueue(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, bool deferred,
+// guard placement of a batch run: 0 inline, 1 deferred (phases 1 + 2), 3 served
+static hipError_t batch_enqueue(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, int form,
                                 int block, hipStream_t s) {
   hipError_t e = launch_fill_u64(b->d_work, 0ull, 4 * b->n_items, s);
   unsigned long long* cnt = (unsigned long long*)counters;
-  if (deferred) {
+  const bool deferred = form == 1;
+  if (form == 3) {
+    const int g3 = ctx->n_cu * batched_blocks_per_cu(3, block, b->slice_floats);
+    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 3, g3, block, s);
+  } else if (deferred) {
     const int g1 = ctx->n_cu * batched_blocks_per_cu(1, block, b->slice_floats);
     const int g2 = ctx->n_cu * batched_blocks_per_cu(2, block, b->slice_floats);
     if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 1, g1, block, s);
@@ -769,26 +774,30 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
   if (launch && launch->precision == MCD_PREC_FP64)
     return set_err(ctx, MCD_ERR_UNSUPPORTED, "batched launches run the FP32 transport");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  const int block = launch && launch->block > 0 ? launch->block : 256;
-  if (block % 64 || block > 256) return set_err(ctx, MCD_ERR_INVALID, "block %d invalid", block);
   const int flags = launch ? launch->flags : 0;
-  if ((flags & MCD_LAUNCH_INLINE_GUARD) && (flags & MCD_LAUNCH_DEFER_GUARD))
-    return set_err(ctx, MCD_ERR_INVALID, "MCD_LAUNCH_INLINE_GUARD and MCD_LAUNCH_DEFER_GUARD both set");
-  if (flags & MCD_LAUNCH_SERVICE_GUARD)
-    return set_err(ctx, MCD_ERR_UNSUPPORTED, "the served guard is built for mcd_forward; batches run inline or deferred");
-  // DS items defer the FP64 guard pick to a second phase only when asked to
-  const bool deferred = b->has_ds && (flags & MCD_LAUNCH_DEFER_GUARD);
+  const int guard_flags = flags & (MCD_LAUNCH_INLINE_GUARD | MCD_LAUNCH_DEFER_GUARD | MCD_LAUNCH_SERVICE_GUARD);
+  if (guard_flags & (guard_flags - 1))
+    return set_err(ctx, MCD_ERR_INVALID, "at most one of MCD_LAUNCH_INLINE_GUARD, _DEFER_GUARD, _SERVICE_GUARD");
+  // DS items take the FP64 guard pick inline unless another placement is asked for
+  const int form = !b->has_ds ? 0 : ((flags & MCD_LAUNCH_DEFER_GUARD) ? 1 : ((flags & MCD_LAUNCH_SERVICE_GUARD) ? 3 : 0));
+  const int block = launch && launch->block > 0 ? launch->block : 256;
+  if (form == 3) {
+    if (block % 64 || block < 128 || block > 1024)
+      return set_err(ctx, MCD_ERR_INVALID, "served guard: block %d must be a multiple of 64 in [128, 1024]", block);
+  } else if (block % 64 || block > 256) {
+    return set_err(ctx, MCD_ERR_INVALID, "block %d invalid", block);
+  }
   hipStream_t s = (hipStream_t)stream;
   if (!use_graph) {
-    HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, deferred, block, s));
+    HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, form, block, s));
     return MCD_OK;
   }
   if (!b->graph || b->graph_seed != seed || b->graph_counters != counters || b->graph_block != block ||
-      b->graph_deferred != (int)deferred) {
+      b->graph_deferred != form) {
     if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
     if (!b->capture_stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
     HIP_TRY(ctx, hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = batch_enqueue(ctx, b, seed, counters, deferred, block, b->capture_stream);
+    hipError_t e = batch_enqueue(ctx, b, seed, counters, form, block, b->capture_stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
     HIP_TRY(ctx, e);
@@ -799,7 +808,7 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
     b->graph_seed = seed;
     b->graph_counters = counters;
     b->graph_block = block;
-    b->graph_deferred = (int)deferred;
+    b->graph_deferred = form;
   }
   HIP_TRY(ctx, hipGraphLaunch(b->graph, s));
   return MCD_OK;

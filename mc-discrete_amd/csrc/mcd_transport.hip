@@ -573,27 +573,112 @@ int batched_cfg_floats() { return kCfgFloats; }
 
 template <int MODE, int NCH, int PHASE>
 __device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, const float* pick, uint64_t seed,
-                                         uint32_t& ne, uint32_t& nf) {
+                                         uint32_t& ne, uint32_t& nf, GuardBox* box, uint32_t* pending) {
   RunArgs R = itR;
   R.seed = seed;
   // c is the wavefront's LDS copy, so its two table descriptors are in LDS too
-  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0>(c, pick, &c.tm, R, ne, nf);
+  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0>(c, pick, &c.tm, R, ne, nf, box, pending);
+}
+
+// The batched kernel's service wavefront: as guard_service, for mailboxes
+// whose transport wavefronts may be on different configurations; the
+// configuration of mailbox w is the LDS copy in wavefront w + 1's slice (that
+// wavefront waits for the reply, so the copy stays in place).
+__device__ void guard_service_batched(const float* lds, int slice_floats, GuardBox* boxes, uint32_t* pending,
+                                      uint32_t* done, int n_boxes) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t nb = (uint32_t)__builtin_amdgcn_readfirstlane(n_boxes);
+  __builtin_amdgcn_s_setprio(3);
+  uint32_t idle = 0;
+  while (idle < kServiceIdleCap) {
+    if (__builtin_amdgcn_readfirstlane(box_load(pending)) == 0u) {
+      if (__builtin_amdgcn_readfirstlane(box_load(done)) == nb) break;
+      __builtin_amdgcn_s_sleep(8);
+      ++idle;
+      continue;
+    }
+    uint32_t q = 0;
+    if (lane == 0) q = __hip_atomic_exchange(pending, 0u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    q = __builtin_amdgcn_readfirstlane(q);
+    while (q) {
+      const int w = __builtin_ctz(q);
+      GuardBox& b = boxes[w];
+      q &= q - 1u;
+      if (box_load(&b.state) != kBoxPosted) continue;
+      const DevCfg& c = *reinterpret_cast<const DevCfg*>(lds + (size_t)(w + 1) * slice_floats);
+      const int nch = __builtin_amdgcn_readfirstlane(c.n_shells + 3);
+      if ((b.mask >> lane) & 1ull) {
+        GuardSlot& g = b.lane[lane];
+        const int meta = g.meta;
+        const double E = (double)g.Ehi + (double)g.Elo, u = (double)g.u;
+        const int sw = meta & 0xFF;
+        const bool ex = (meta >> 8) & 1;
+        int ch;
+        switch (nch) {
+          case 4: ch = d_pick_ordered<4>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+          case 5: ch = d_pick_ordered<5>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+          case 6: ch = d_pick_ordered<6>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+          case 7: ch = d_pick_ordered<7>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+          case 8: ch = d_pick_ordered<8>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+          case 9: ch = d_pick_ordered<9>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+          case 10: ch = d_pick_ordered<10>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+          default: ch = d_pick_ordered<11>(E, u, c, g.perm_lo, g.perm_hi, sw, ex); break;
+        }
+        g.meta = ch;
+      }
+      box_store(&b.state, kBoxReplied);
+    }
+  }
+}
+
+// LDS of the batched kernel: one slice per wavefront, then (served guard)
+// one mailbox per transport wavefront and the finished count and pending bits.
+__host__ __device__ inline size_t batched_box_offset(int block, int slice_floats) {
+  return ((size_t)(block / 64) * slice_floats * sizeof(float) + 15) & ~(size_t)15;
+}
+static size_t batched_lds_bytes(int phase, int block, int slice_floats) {
+  if (phase != 3) return (size_t)(block / 64) * slice_floats * sizeof(float);
+  return batched_box_offset(block, slice_floats) + (size_t)(block / 64 - 1) * sizeof(GuardBox) + 16;
 }
 
 // PHASE 0: guard inline.  PHASE 1: DS items park guarded steps (no FP64 code
-// in the kernel).  PHASE 2: DS items resume their parked electrons.
+// in the kernel).  PHASE 2: DS items resume their parked electrons.  PHASE 3:
+// wavefront 0 of each workgroup serves the guard picks of the others (no FP64
+// code on the transport path, one phase).
 // Register budgets measured on MI355X for the 64-configuration batch: 4
 // resident wavefronts per SIMD for phases 0 and 2, 5 for phase 1 (the FP64-free
 // phase), each faster than the compiler's own allocation despite small spills.
 template <int PHASE>
-__global__ __launch_bounds__(256, PHASE == 1 ? 5 : 4) void k_transport_batched(const BatchItemDev* __restrict__ items, int n_items,
-                                                            int slice_floats, uint64_t seed,
-                                                            unsigned long long* __restrict__ counters) {
+__global__ __launch_bounds__(1024, PHASE == 1 || PHASE == 3 ? 5 : 4) void k_transport_batched(
+    const BatchItemDev* __restrict__ items, int n_items, int slice_floats, uint64_t seed,
+    unsigned long long* __restrict__ counters) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & 63;
-  const int wib = threadIdx.x >> 6;
+  const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   float* my = lds + (size_t)wib * slice_floats;
   float* pick = my + kCfgFloats;
+  GuardBox* boxes = nullptr;
+  uint32_t *done = nullptr, *pending = nullptr;
+  if constexpr (PHASE == 3) {
+    const int n_boxes = (int)(blockDim.x >> 6) - 1;
+    boxes = reinterpret_cast<GuardBox*>(reinterpret_cast<char*>(lds) + batched_box_offset(blockDim.x, slice_floats));
+    done = reinterpret_cast<uint32_t*>(boxes + n_boxes);
+    pending = done + 1;
+    if (threadIdx.x == 0) {
+      *done = 0u;
+      *pending = 0u;
+    }
+    if ((int)threadIdx.x < n_boxes) {
+      boxes[threadIdx.x].state = 0u;
+      boxes[threadIdx.x].bit = 1u << threadIdx.x;
+    }
+    __syncthreads();
+    if (wib == 0) {
+      guard_service_batched(lds, slice_floats, boxes, pending, done, n_boxes);
+      return;
+    }
+  }
+  GuardBox* box = PHASE == 3 ? boxes + (wib - 1) : nullptr;
   int ci = -1;
   if (PHASE != 2) {
     const int waves_total = gridDim.x * (blockDim.x >> 6);
@@ -621,17 +706,17 @@ __global__ __launch_bounds__(256, PHASE == 1 ? 5 : 4) void k_transport_batched(c
     const DevCfg& c = *reinterpret_cast<const DevCfg*>(my);
     uint32_t ne = 0, nf = 0;
     if (mode == MCD_MODE_CONT) {
-      if (PHASE != 2) run_item<MCD_MODE_CONT, 4, PHASE>(c, it.R, pick, seed, ne, nf);
+      if (PHASE != 2) run_item<MCD_MODE_CONT, 4, PHASE>(c, it.R, pick, seed, ne, nf, nullptr, nullptr);
     } else {
       switch (nch) {
-        case 4: run_item<MCD_MODE_DS, 4, PHASE>(c, it.R, pick, seed, ne, nf); break;
-        case 5: run_item<MCD_MODE_DS, 5, PHASE>(c, it.R, pick, seed, ne, nf); break;
-        case 6: run_item<MCD_MODE_DS, 6, PHASE>(c, it.R, pick, seed, ne, nf); break;
-        case 7: run_item<MCD_MODE_DS, 7, PHASE>(c, it.R, pick, seed, ne, nf); break;
-        case 8: run_item<MCD_MODE_DS, 8, PHASE>(c, it.R, pick, seed, ne, nf); break;
-        case 9: run_item<MCD_MODE_DS, 9, PHASE>(c, it.R, pick, seed, ne, nf); break;
-        case 10: run_item<MCD_MODE_DS, 10, PHASE>(c, it.R, pick, seed, ne, nf); break;
-        default: run_item<MCD_MODE_DS, 11, PHASE>(c, it.R, pick, seed, ne, nf); break;
+        case 4: run_item<MCD_MODE_DS, 4, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        case 5: run_item<MCD_MODE_DS, 5, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        case 6: run_item<MCD_MODE_DS, 6, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        case 7: run_item<MCD_MODE_DS, 7, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        case 8: run_item<MCD_MODE_DS, 8, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        case 9: run_item<MCD_MODE_DS, 9, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        case 10: run_item<MCD_MODE_DS, 10, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        default: run_item<MCD_MODE_DS, 11, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
       }
     }
     const uint64_t cnt[2] = {ne, nf};
@@ -642,6 +727,10 @@ __global__ __launch_bounds__(256, PHASE == 1 ? 5 : 4) void k_transport_batched(c
     ci = next_item<PHASE>(items, n_items, lane);
     __builtin_amdgcn_wave_barrier();
   }
+  if constexpr (PHASE == 3) {
+    // every lane is converged here; the service wavefront stops once all are counted
+    if (lane == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
 }
 
 template <int PHASE>
@@ -651,12 +740,16 @@ static const void* batched_kernel() {
 
 hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
                           unsigned long long* counters, int phase, int grid, int block, hipStream_t s) {
-  const size_t lds = (size_t)(block / 64) * slice_floats * sizeof(float);
+  if (phase == 3 && (block < 128 || block > 1024)) return hipErrorInvalidValue;
+  const size_t lds = batched_lds_bytes(phase, block, slice_floats);
   if (phase == 1)
     hipLaunchKernelGGL(k_transport_batched<1>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
                        counters);
   else if (phase == 2)
     hipLaunchKernelGGL(k_transport_batched<2>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
+                       counters);
+  else if (phase == 3)
+    hipLaunchKernelGGL(k_transport_batched<3>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
                        counters);
   else
     hipLaunchKernelGGL(k_transport_batched<0>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
@@ -666,8 +759,9 @@ hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floa
 
 int batched_blocks_per_cu(int phase, int block, int slice_floats) {
   int nb = 0;
-  const size_t lds = (size_t)(block / 64) * slice_floats * sizeof(float);
-  const void* k = phase == 1 ? batched_kernel<1>() : (phase == 2 ? batched_kernel<2>() : batched_kernel<0>());
+  const size_t lds = batched_lds_bytes(phase, block, slice_floats);
+  const void* k = phase == 1 ? batched_kernel<1>()
+                             : (phase == 2 ? batched_kernel<2>() : (phase == 3 ? batched_kernel<3>() : batched_kernel<0>()));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess || nb < 1) nb = 1;
   return nb;
 }

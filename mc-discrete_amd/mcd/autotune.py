@@ -21,8 +21,10 @@ The batched kernel has three shape choices that trade against each other:
   fit a CU; 0 sizes each table to the default 8 KB slice.
 * ``block`` — threads per workgroup (64, 128 or 256).
 * ``guard`` — where DS items take the FP64 guard pick: ``"deferred"`` (a
-  first phase without FP64 code parks those electrons, a second resumes them)
-  or ``"inline"`` (one phase, more registers, fewer resident wavefronts).
+  first phase without FP64 code parks those electrons, a second resumes them),
+  ``"inline"`` (one phase, more registers, fewer resident wavefronts) or
+  ``"service"`` (one phase; wavefront 0 of each workgroup settles the picks of
+  the others, which carry no FP64 code).
 
 The batched kernel refills idle lanes every step (ballot + prefix count).
 
@@ -41,6 +43,7 @@ import os
 import time
 
 CANDIDATES = (
+    dict(block=1024, pick_nodes=128, guard="service"),
     dict(block=128, pick_nodes=128, guard="inline"),
     dict(block=256, pick_nodes=128, guard="inline"),
     dict(block=256, pick_nodes=0, guard="inline"),

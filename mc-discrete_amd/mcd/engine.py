@@ -326,7 +326,7 @@ class Batch:
             pass
 
     def launch(self, seed=3, use_graph=True, block=0, guard=None, stream=None):
-        """guard: None (library default: inline), "inline" or "deferred"."""
+        """guard: None (library default: inline), "inline", "deferred" or "service"."""
         ctx = context(self.device)
         la = _abi.Launch(int(block), 0, _abi.PREC_FP32, guard_flags(guard))
         s = (stream or torch.cuda.current_stream(self.device)).cuda_stream

@@ -25,7 +25,8 @@ def small_batch():
     return cfgs, trs
 
 
-@pytest.mark.parametrize("use_graph,guard", [(False, None), (True, None), (True, "deferred")])
+@pytest.mark.parametrize("use_graph,guard", [(False, None), (True, None), (True, "deferred"), (False, "service"),
+                                             (True, "service")])
 def test_batched_equals_per_configuration_launches(small_batch, use_graph, guard):
     from mcd import _abi, engine
     cfgs, trs = small_batch
@@ -41,15 +42,20 @@ def test_batched_equals_per_configuration_launches(small_batch, use_graph, guard
             assert int(r.counters[_abi.CNT_ELECTRONS]) == c["n"]
 
 
-def test_batch_rejects_the_served_guard(small_batch):
-    """The served guard is built for single-configuration launches; a batch
-    asked for it reports MCD_ERR_UNSUPPORTED instead of launching."""
+def test_served_batch_at_several_workgroup_sizes(small_batch):
+    """The served guard in the batched kernel (wavefront 0 of each workgroup
+    settles the guard picks of the other wavefronts, whatever configuration
+    each is on) gives the per-configuration records at 2 to 16 wavefronts per
+    workgroup."""
     from mcd import engine
-    from mcd._lib import MCDError
     cfgs, trs = small_batch
-    b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs[:2], cfgs[:2])], pick_nodes=128)
-    with pytest.raises(MCDError, match="served guard"):
-        b.launch(seed=3, use_graph=False, guard="service")
+    b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs, cfgs)], pick_nodes=128)
+    refs = [tr.run(c["n"], seed=3, config_id=i).els.tobytes() for i, (tr, c) in enumerate(zip(trs, cfgs))]
+    for block in (128, 640, 1024):
+        b.counters.zero_()
+        b.launch(seed=3, use_graph=False, block=block, guard="service")
+        for i, r in enumerate(b.results()):
+            assert r.els.tobytes() == refs[i], (block, i)
 
 
 def test_autotuner_picks_and_caches_a_launch_shape(small_batch, tmp_path):
