@@ -217,7 +217,7 @@ __device__ void guard_service(const DevCfg& c, GuardBox* boxes, uint32_t* pendin
 // same records as phase 0: a parked step is redone from the same state and
 // Philox counter.  PHASE 3 (DS): guard-band picks go to the workgroup's
 // service wavefront through `box`; same records as phase 0.
-template <int MODE, int NCH, int PHASE = 0>
+template <int MODE, int NCH, int PHASE = 0, bool PERIOD = false>
 __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds,
                                                const DevTable* __restrict__ tabs, const RunArgs& R,
                                                uint32_t& n_events, uint32_t& n_fp64, GuardBox* box = nullptr,
@@ -235,14 +235,15 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
   float p[3] = {0.f, 0.f, 0.f}, pc[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
 
   uint32_t iter = 0;
+  const uint32_t refill_mask = PERIOD ? R.refill_mask : 0u;
   while (true) {
     // the key words stay loop-carried SGPR values (set where all lanes are
     // converged): otherwise the compiler hoists all twenty round keys out of
     // the loop and spills them to VGPR lanes, paying a v_readlane per round
     asm volatile("" : "+s"(k0), "+s"(k1));
-    // compaction period: idle lanes take new electrons every
+    // compaction period (PERIOD kernels): idle lanes take new electrons every
     // (refill_mask + 1) steps, and at once when every lane is idle
-    if (open && ((iter++ & R.refill_mask) == 0u || !__any(active))) {
+    if (open && (!PERIOD || (iter++ & refill_mask) == 0u || !__any(active))) {
       uint64_t idle = __ballot(!active);
       if (idle) {
         const uint64_t idx = refill_index(wq, PHASE == 2 ? R.resume_queue : R.queue, idle, lane, n_work, open);
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
     __syncthreads();
   }
   uint32_t n_events = 0, n_fp64 = 0;
-  transport_loop<MODE, NCH, PHASE>(c, lds, tabs, R, n_events, n_fp64);
+  transport_loop<MODE, NCH, PHASE, true>(c, lds, tabs, R, n_events, n_fp64);
   const int lane = threadIdx.x & 63;
   const uint64_t cnt[2] = {n_events, n_fp64};
   wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(1024, MINW) void k_transport_svc(DevCfg c, RunArgs 
     return;
   }
   uint32_t n_events = 0, n_fp64 = 0;
-  transport_loop<MCD_MODE_DS, NCH, 3>(c, lds, tabs, R, n_events, n_fp64, boxes + (w - 1), pending);
+  transport_loop<MCD_MODE_DS, NCH, 3, true>(c, lds, tabs, R, n_events, n_fp64, boxes + (w - 1), pending);
   const int lane = threadIdx.x & 63;
   // every lane is converged here; the service wavefront stops once all are counted
   if (lane == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -649,7 +650,7 @@ static size_t batched_lds_bytes(int phase, int block, int slice_floats) {
 // resident wavefronts per SIMD for phases 0 and 2, 5 for phase 1 (the FP64-free
 // phase), each faster than the compiler's own allocation despite small spills.
 template <int PHASE>
-__global__ __launch_bounds__(1024, PHASE == 1 || PHASE == 3 ? 5 : 4) void k_transport_batched(
+__global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ? 5 : 4) void k_transport_batched(
     const BatchItemDev* __restrict__ items, int n_items, int slice_floats, uint64_t seed,
     unsigned long long* __restrict__ counters) {
   extern __shared__ float lds[];
