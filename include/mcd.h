@@ -146,7 +146,7 @@ typedef struct mcd_electron_rec {
 /* One `scats` row (scattering.py:527-679, electron.py:59,99), appended to a
  * device buffer in arbitrary order; sort by (config, electron, step). */
 typedef struct mcd_scatter_rec {
-  uint32_t electron;
+  uint32_t electron; /* low 32 bits of the electron id */
   uint32_t step;
   uint16_t config;
   uint8_t channel;   /* MCD_CH_* */
@@ -251,7 +251,8 @@ int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap_cells, int* n
 /* Run electrons [first, first+n) of one configuration.  els: device array of
  * n records (index = electron - first).  scats may be NULL.  counters: device
  * array of MCD_NCOUNTERS uint64, accumulated into (not cleared).  Philox key =
- * seed, counter = (electron, config_id, step, 0).  Asynchronous on stream. */
+ * seed, counter = (electron low 32 bits, config_id | electron bits 32..47 << 16,
+ * step, 0), so first + n must not exceed 2^48.  Asynchronous on stream. */
 int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
                 uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
                 const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch,

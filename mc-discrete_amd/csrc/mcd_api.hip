@@ -478,6 +478,9 @@ extern "C" int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap, i
 // Smallest DS FP32 launch that takes the served guard when the caller names
 // neither a guard placement nor a launch shape.
 static const uint64_t kServedDefaultMin = 1ull << 21;
+// Electron ids are 48 bits: the low 32 are Philox counter word 0, the high 16
+// share word 1 with the 16-bit configuration id.
+static const uint64_t kMaxElectronId = 1ull << 48;
 
 static int default_minw(const mcd_config&) {
   const char* s = getenv("MCD_MINW");
@@ -500,6 +503,8 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   if (n == 0) return MCD_OK;
   if (!els) return set_err(ctx, MCD_ERR_INVALID, "mcd_forward: els is NULL");
   if (config_id > 0xFFFFu) return set_err(ctx, MCD_ERR_INVALID, "config_id must fit 16 bits");
+  if (first > kMaxElectronId || n > kMaxElectronId - first)
+    return set_err(ctx, MCD_ERR_INVALID, "electrons [first, first+n) must lie below 2^48");
   const int precision = launch ? launch->precision : MCD_PREC_FP32;
   if (cfg->mode == MCD_MODE_DS) {
     if (!tables || !tables->has_w) return set_err(ctx, MCD_ERR_TABLE, "DS mode needs tables from mcd_tables_create");
@@ -661,6 +666,10 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
     int rc = check_config(ctx, it.cfg);
     if (rc) { mcd_batch_destroy(b); return rc; }
     if (!it.els && it.n) { mcd_batch_destroy(b); return set_err(ctx, MCD_ERR_INVALID, "item %d: els is NULL", i); }
+    if (it.n > kMaxElectronId) {
+      mcd_batch_destroy(b);
+      return set_err(ctx, MCD_ERR_INVALID, "item %d: electron count above 2^48", i);
+    }
     mcd_tables* own = nullptr;
     if (it.cfg->mode == MCD_MODE_DS) {
       if (!it.tables || !it.tables->has_w || !same_config(*it.cfg, it.tables->cfg)) {
@@ -913,6 +922,8 @@ extern "C" int mcd_backward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table
     return set_err(ctx, MCD_ERR_INVALID, "mcd_backward: pass exactly one of out_jac and out_vjp");
   if (out_vjp && !grad_out) return set_err(ctx, MCD_ERR_INVALID, "mcd_backward: out_vjp needs grad_out");
   if (config_id > 0xFFFFu) return set_err(ctx, MCD_ERR_INVALID, "config_id must fit 16 bits");
+  if (first > kMaxElectronId || n > kMaxElectronId - first)
+    return set_err(ctx, MCD_ERR_INVALID, "electrons [first, first+n) must lie below 2^48");
   if (!tables || !same_config(*cfg, tables->cfg) || tables->cfg.mode != cfg->mode)
     return set_err(ctx, MCD_ERR_TABLE, "mcd_backward needs tables built for this configuration");
   if (cfg->mode == MCD_MODE_DS && !tables->has_w)

@@ -97,6 +97,14 @@ SCATTER_DTYPE = np.dtype([
 assert SCATTER_DTYPE.itemsize == 64
 
 
+def row_offsets(rows, first):
+    """Electron index of each scatter row within a launch that began at
+    electron `first` (int64).  A row holds the low 32 bits of the electron id,
+    so the offset is their difference modulo 2^32 (a launch spans fewer than
+    2^32 electrons)."""
+    return (rows["electron"] - np.uint32(first & 0xFFFFFFFF)).astype(np.int64)
+
+
 def make_config(mat, E0, Emin, Wc=50.0, s_tilt=0.0, thickness=float("inf"), mode="DS",
                 bethe_model="classical", max_scatt=0, J=None) -> Config:
     """Fill an mcd_config from a mcd.material.Material and run parameters."""

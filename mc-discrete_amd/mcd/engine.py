@@ -200,7 +200,7 @@ class Transport:
             if used > scat_capacity:
                 raise ScatterOverflow(f"scatter-row buffer overflow: {used} rows requested, capacity {scat_capacity}")
             srows = rows.cpu().numpy()[: used * _abi.SCATTER_DTYPE.itemsize].view(_abi.SCATTER_DTYPE)
-            srows = srows[np.lexsort((srows["step"], srows["electron"]))]
+            srows = srows[np.lexsort((srows["step"], _abi.row_offsets(srows, first)))]
         return RunResult(els=e, counters=counters, scats=srows)
 
 

@@ -126,9 +126,9 @@ def per_electron_dicts(els: np.ndarray, rows, material, things, first=0):
     sc_want = list(things.get("scat_output", ()))
     names = channel_names(material)
     if rows is not None and sc_want:
-        ids = rows["electron"].astype(np.int64)
-        lo = np.searchsorted(ids, first + np.arange(els.size), side="left")
-        hi = np.searchsorted(ids, first + np.arange(els.size), side="right")
+        ids = _abi.row_offsets(rows, first)
+        lo = np.searchsorted(ids, np.arange(els.size), side="left")
+        hi = np.searchsorted(ids, np.arange(els.size), side="right")
     out = []
     for i, e in enumerate(els):
         ed = {}

@@ -32,12 +32,13 @@ M0, M1, W0, W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
 
 
 def philox_block(electron, config, steps, seed):
-    """Philox4x32-10 for counters (electron, config, step, 0), step in `steps`;
-    returns (len(steps), 4) uniforms (2*(x>>9)+1)*2^-24."""
+    """Philox4x32-10 for counters (electron low 32 bits, config | electron
+    bits 32..47 << 16, step, 0), step in `steps`; returns (len(steps), 4)
+    uniforms (2*(x>>9)+1)*2^-24."""
     n = len(steps)
     mask = np.uint64(0xFFFFFFFF)
     c0 = np.full(n, electron & 0xFFFFFFFF, dtype=np.uint64)
-    c1 = np.full(n, config, dtype=np.uint64)
+    c1 = np.full(n, config | (electron >> 32) << 16, dtype=np.uint64)
     c2 = np.asarray(steps, dtype=np.uint64)
     c3 = np.zeros(n, dtype=np.uint64)
     k0, k1 = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF

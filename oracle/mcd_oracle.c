@@ -7,7 +7,8 @@
  *
  * It restates, in plain C and FP64, the reference's per-step semantics with the
  * reference's random draws replaced by Philox4x32-10 (key = seed, counter =
- * (electron, config, step, 0)), one call per step:
+ * (electron low 32 bits, config | electron bits 32..47 << 16, step, 0)), one
+ * call per step:
  *   DS   slots [0]=channel pick, [1]=free path, [2]=W or Rutherford angle, [3]=azimuth
  *   cont slots [0]=free path, [1]=Rutherford angle, [2]=azimuth
  * Every function cites the reference lines it follows (paths relative to the
@@ -61,7 +62,7 @@ static double or_u01(uint32_t x) { return (double)(2u * (x >> 9) + 1u) * 5.96046
 double oracle_u01(uint32_t x) { return or_u01(x); }
 
 static void or_draw(uint64_t seed, uint64_t electron, uint32_t config, uint32_t step, double u[4]) {
-  uint32_t ctr[4] = {(uint32_t)electron, config, step, 0u};
+  uint32_t ctr[4] = {(uint32_t)electron, config | (uint32_t)(electron >> 32) << 16, step, 0u};
   uint32_t o[4];
   or_philox(ctr, (uint32_t)seed, (uint32_t)(seed >> 32), o);
   for (int i = 0; i < 4; ++i) u[i] = or_u01(o[i]);
@@ -427,6 +428,8 @@ int oracle_run(const mcd_config* c, const oracle_table* tm, const oracle_table* 
                uint64_t trace_cap, uint64_t* trace_n, uint64_t* events, int nthreads) {
   if (!c || !els || !events) return MCD_ERR_INVALID;
   if (c->mode == MCD_MODE_DS && (!tm || !tg)) return MCD_ERR_INVALID;
+  /* the counter holds 48 electron bits beside a 16-bit configuration id */
+  if (config_id > 0xFFFFu || first > (1ull << 48) || n > (1ull << 48) - first) return MCD_ERR_INVALID;
   uint64_t ev = 0;
   if (trace || nthreads <= 1) {
     uint64_t tn = trace_n ? *trace_n : 0;

@@ -233,6 +233,29 @@ def test_split_launches_equal_one_launch(cu20):
     assert both.tobytes() == full.els.tobytes()
 
 
+def test_electron_ids_across_2_32_match_oracle(si10):
+    """Electrons 2^32 - 24 .. 2^32 + 23: the FP64 kernel equals the oracle bit
+    for bit (bits 32..47 of the id in Philox counter word 1), every guard form
+    gives the same FP32 records, the scatter rows (which hold the low 32 bits
+    of the id) come back in electron order across the wrap and equal the
+    oracle's trace, and a range past 2^48 is refused before launch."""
+    tr, tm, tg = si10
+    first, n = (1 << 32) - 24, 48
+    o, tro, ev = O.run(tr.cfg, tm, tg, 6, first, n, trace_cap=200000)
+    r = tr.run(n, seed=6, first=first, precision=_abi.PREC_FP64)
+    assert identical(r.els, o).all() and r.events == ev
+    check_values(r.els, o, identical(r.els, o), rel_E=1e-7, dir_abs=1e-7, pos_rel=1e-7)
+    base = tr.run(n, seed=6, first=first, guard="inline")
+    for guard in ("deferred", "service"):
+        assert tr.run(n, seed=6, first=first, guard=guard).els.tobytes() == base.els.tobytes(), guard
+    s = tr.run(n, seed=6, first=first, precision=_abi.PREC_FP64, scats=True).scats
+    assert len(s) == len(tro)
+    assert np.array_equal(s["electron"], tro["electron"]) and np.array_equal(s["step"], tro["step"])
+    assert np.array_equal(s["channel"], tro["channel"])
+    with pytest.raises(RuntimeError, match="2\\^48"):
+        tr.run(8, first=(1 << 48) - 4)
+
+
 @pytest.mark.parametrize("n", [1, 15, 17, 1000, (1 << 20) + 3])
 def test_every_electron_runs_once_at_any_count(cu20, n):
     """Wavefronts take electron indices in blocks of 16 per atomic: for counts

@@ -138,6 +138,28 @@ def test_oracle_matches_independent_python_restatement(si10, mode, bethe):
         assert np.allclose(r["els"]["last_pos"][0], e["pos"], rtol=1e-10, atol=1e-8)
 
 
+def test_electron_ids_above_2_32_have_streams_of_their_own(si10):
+    """Electron ids are 48 bits: bits 32..47 go into Philox counter word 1
+    beside the configuration id.  Electron 2^32 + k must not replay electron
+    k's stream, the two implementations agree across the 2^32 boundary, and
+    ranges that reach past 2^48 are refused."""
+    cfg = _abi.make_config(si10["mat"], 10000, 1000, 100, 70)
+    lo, _, _ = O.run(cfg, si10["tm"], si10["tg"], 3, 0, 32)
+    hi, _, _ = O.run(cfg, si10["tm"], si10["tg"], 3, 1 << 32, 32)
+    same = (lo["n_scatt"] == hi["n_scatt"]) & (lo["final_E"] == hi["final_E"])
+    assert same.sum() <= 1, same.sum()
+    first = (1 << 32) - 4
+    els, _, ev = O.run(cfg, si10["tm"], si10["tg"], 3, first, 8)
+    ev1, res = B1.run_serial(B1.params_from(cfg, si10["tm"], si10["tg"], seed=3), first, 8)
+    assert ev1 == ev
+    for r, e in zip(res, els):
+        assert r["els"]["final_E"][0] == pytest.approx(e["final_E"], rel=1e-12)
+    assert np.array_equal(els[4:], hi[:4])
+    O.run(cfg, si10["tm"], si10["tg"], 3, (1 << 48) - 2, 2)
+    with pytest.raises(RuntimeError):
+        O.run(cfg, si10["tm"], si10["tg"], 3, (1 << 48) - 2, 3)
+
+
 def test_trace_invariants(si10):
     cfg = _abi.make_config(si10["mat"], 10000, 1000, 100, 70)
     els, tr, ev = O.run(cfg, si10["tm"], si10["tg"], 0, 0, 64, trace_cap=200000)
