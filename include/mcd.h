@@ -278,7 +278,8 @@ int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n_items, int
 /* Run every item in one persistent launch.  counters: device array of
  * n_items x MCD_NCOUNTERS uint64 (accumulated into).  use_graph != 0 records
  * the launch sequence into a hipGraph once (per seed, counters and stream
- * shape) and replays it.  Asynchronous on stream. */
+ * shape) and replays it.  Items may hold 0 electrons (no wavefront takes
+ * them; their counters stay untouched).  Asynchronous on stream. */
 int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, const mcd_launch* launch,
                   int use_graph, void* stream);
 int mcd_batch_destroy(mcd_batch* b);

@@ -787,8 +787,11 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
   const int guard_flags = flags & (MCD_LAUNCH_INLINE_GUARD | MCD_LAUNCH_DEFER_GUARD | MCD_LAUNCH_SERVICE_GUARD);
   if (guard_flags & (guard_flags - 1))
     return set_err(ctx, MCD_ERR_INVALID, "at most one of MCD_LAUNCH_INLINE_GUARD, _DEFER_GUARD, _SERVICE_GUARD");
-  // DS items take the FP64 guard pick inline unless another placement is asked for
-  const int form = !b->has_ds ? 0 : ((flags & MCD_LAUNCH_DEFER_GUARD) ? 1 : ((flags & MCD_LAUNCH_SERVICE_GUARD) ? 3 : 0));
+  // DS items take the FP64 guard pick inline unless another placement is asked
+  // for; the served form keeps its workgroup shape when no DS item has
+  // electrons (its service wavefronts then stay idle), the deferred form then
+  // runs as the inline one
+  const int form = (flags & MCD_LAUNCH_SERVICE_GUARD) ? 3 : ((b->has_ds && (flags & MCD_LAUNCH_DEFER_GUARD)) ? 1 : 0);
   const int block = launch && launch->block > 0 ? launch->block : 256;
   if (form == 3) {
     if (block % 64 || block < 128 || block > 1024)

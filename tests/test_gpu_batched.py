@@ -85,3 +85,32 @@ def test_autotuner_picks_and_caches_a_launch_shape(small_batch, tmp_path):
     for i, (tr, c, r) in enumerate(zip(trs, cfgs, res)):
         assert r.els.tobytes() == tr.run(c["n"], seed=3, config_id=i).els.tobytes(), i
     assert os.path.exists(path)
+
+
+@pytest.mark.parametrize("guard", [None, "deferred", "service"])
+def test_ragged_batch_with_empty_and_single_electron_items(small_batch, guard):
+    """Ragged items (0, 1 and 17 electrons beside larger ones, DS and
+    continuous) give the per-configuration records and counts; empty items
+    are never assigned a wavefront and count nothing, and a batch whose items
+    are all empty ends with zero counters, eagerly and as a graph replay."""
+    from mcd import _abi, engine
+    cfgs, trs = small_batch
+    sizes = [0, 1, 17, 0, 300, 5, 0, 1000]
+    picks = [0, 9, 18, 27, 36, 45, 54, 63]  # DS and continuous items of all four materials
+    modes = {cfgs[k]["mode"] for k in picks}
+    assert len(picks) == len(sizes) and len(modes) == 2, modes
+    items = [(trs[k], n) for k, n in zip(picks, sizes)]
+    b = engine.Batch(items, pick_nodes=128)
+    for use_graph in (False, True):
+        b.counters.zero_()
+        b.launch(seed=8, use_graph=use_graph, guard=guard, block=640 if guard == "service" else 0)
+        for i, ((tr, n), r) in enumerate(zip(items, b.results())):
+            ref = tr.run(n, seed=8, config_id=i)
+            assert r.els.tobytes() == ref.els.tobytes(), (i, n)
+            assert int(r.counters[_abi.CNT_ELECTRONS]) == n, (i, n)
+            assert int(r.counters[_abi.CNT_EVENTS]) == (ref.events if n else 0), (i, n)
+    empty = engine.Batch([(trs[k], 0) for k in picks[:3]], pick_nodes=128)
+    for use_graph in (False, True):
+        empty.counters.zero_()
+        empty.launch(seed=8, use_graph=use_graph, guard=guard, block=640 if guard == "service" else 0)
+        assert all(not r.counters.any() for r in empty.results())
