@@ -188,3 +188,20 @@ def test_backward_rejects_unsupported_requests():
         tr._bwd(16, 4, 0, 0, torch.zeros(65, device="cuda"), torch.zeros(5, dtype=torch.float64, device="cuda"),
                 out, 64, 0.0, 0.0, 0, None)
     assert e.value.code == _abi.ERR_INVALID
+
+
+def test_backward_empty_range_and_id_limit():
+    """An empty replay leaves the accumulators untouched; a range past 2^48
+    (the electron ids Philox keys hold) is refused before launch."""
+    import torch
+    from mcd import _abi
+    from mcd._lib import MCDError
+    from mcd.engine import Transport
+    tr = Transport("Cu", 20000.0, 1000.0, Wc=50.0, s_tilt=70.0)
+    out = torch.full((65, 1 + _abi.NGRAD), 7.0, dtype=torch.float64, device="cuda")
+    tr._bwd(0, 4, 0, 0, None, None, out, 64, 0.0, 0.0, 0, None)
+    torch.cuda.synchronize()
+    assert bool((out == 7.0).all())
+    with pytest.raises(MCDError) as e:
+        tr._bwd(8, 4, (1 << 48) - 4, 0, None, None, out, 64, 0.0, 0.0, 0, None)
+    assert e.value.code == _abi.ERR_INVALID

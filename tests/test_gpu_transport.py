@@ -256,12 +256,13 @@ def test_electron_ids_across_2_32_match_oracle(si10):
         tr.run(8, first=(1 << 48) - 4)
 
 
-@pytest.mark.parametrize("n", [1, 15, 17, 1000, (1 << 20) + 3])
+@pytest.mark.parametrize("n", [0, 1, 15, 17, 1000, (1 << 20) + 3])
 def test_every_electron_runs_once_at_any_count(cu20, n):
     """Wavefronts take electron indices in blocks of 16 per atomic: for counts
     that are not multiples of the block (or of a wavefront) every electron is
     transported exactly once -- its record is written and equals the record of
-    the same electron in a larger launch (records are keyed by electron id)."""
+    the same electron in a larger launch (records are keyed by electron id).
+    An empty launch returns at once and leaves the counters untouched."""
     tr = cu20[0]
     for guard in ("inline", "deferred", "service"):
         r = tr.run(n, seed=7, first=3, guard=guard)
