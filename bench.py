@@ -98,6 +98,20 @@ def cpu_baselines(tr, seconds, seed):
     b1 = {"value": ev1 / dt1, "unit": "events/s", "cores": w, "kind": "port",
           "sample": f"Python/NumPy reference-path restatement, Pool({w}) x 4 trajectories per job, "
                     f"electrons 0..{n1 - 1}, {ev1} events in {dt1:.1f} s"}
+    # the same restatement on one worker (the reference's num_workers = 1) and
+    # as the reference ships it (every job carries the tables, one per shell)
+    ns = max(8, int(seconds * 8))
+    t = time.perf_counter()
+    evs, _ = B1.run_serial(P, 0, ns)
+    dts = time.perf_counter() - t
+    nsh = max(4 * threads, n1 // 4)
+    evh, dth, wh = B1.run_pool_shipped(P, nsh, workers=threads, n_gryz_tables=int(tr.cfg.mat.n_shells))
+    b1["one_worker"] = {"value": evs / dts, "cores": 1,
+                        "sample": f"electrons 0..{ns - 1}, {evs} events in {dts:.1f} s"}
+    b1["as_shipped"] = {"value": evh / dth, "cores": wh,
+                        "sample": f"Pool({wh}), one apply_async per 4 trajectories carrying the Moller table and "
+                                  f"{int(tr.cfg.mat.n_shells)} Gryzinski tables, electrons 0..{nsh - 1}, "
+                                  f"{evh} events in {dth:.1f} s"}
     return c_base, b1
 
 

@@ -353,3 +353,36 @@ def run_pool(params, n_electrons, workers=None, per_job=4):
 def _job_events(first, num):
     ev, _ = multi_traj(first, num)
     return ev
+
+
+def _job_events_shipped(params, first, num):
+    """One job as the reference ships it: the job's arguments carry the
+    parameters and every table (functools.partial over multiTraj_DS with
+    `tables`, doScatter.py:229-233, pickled with each apply_async call,
+    doScatter.py:88-90), so the worker loads them per job."""
+    _init_worker(params)
+    ev, _ = multi_traj(first, num)
+    return ev
+
+
+def run_pool_shipped(params, n_electrons, workers=None, per_job=4, n_gryz_tables=1):
+    """Time n_electrons through a Pool of `workers` processes with one
+    apply_async call per job of `per_job` electrons, each carrying the tables:
+    the Moller table and `n_gryz_tables` Gryzinski tables (the reference's
+    tables dict holds one per shell, genTables probTables.py:541-614; the
+    extra ones are copies of the outermost shell's, of the same size).
+    Returns (events, seconds, workers)."""
+    workers = workers or os.cpu_count()
+    shipped = dict(params)
+    if params.get("tg") is not None:
+        # distinct copies, so that pickle writes each one (it would share a
+        # repeated reference to one array)
+        shipped["tg_other_shells"] = [tuple(np.array(a, copy=True) for a in params["tg"])
+                                      for _ in range(max(0, n_gryz_tables - 1))]
+    t = time.perf_counter()
+    with Pool(workers) as pool:
+        res = [pool.apply_async(_job_events_shipped, (shipped, i, min(per_job, n_electrons - i)))
+               for i in range(0, n_electrons, per_job)]
+        ev = sum(r.get() for r in res)
+    dt = time.perf_counter() - t
+    return int(ev), dt, workers

@@ -138,6 +138,18 @@ def test_oracle_matches_independent_python_restatement(si10, mode, bethe):
         assert np.allclose(r["els"]["last_pos"][0], e["pos"], rtol=1e-10, atol=1e-8)
 
 
+def test_baseline_pool_forms_do_the_same_work(si10):
+    """The CPU-baseline drivers (serial, pool with tables loaded once per
+    worker, pool with the tables shipped in every job) transport the same
+    electrons: equal event counts, equal to the oracle's."""
+    cfg = _abi.make_config(si10["mat"], 10000, 1000, 100, 70)
+    _, _, ev = O.run(cfg, si10["tm"], si10["tg"], 2, 0, 12)
+    P = B1.params_from(cfg, si10["tm"], si10["tg"], seed=2)
+    assert B1.run_serial(P, 0, 12)[0] == ev
+    assert B1.run_pool(P, 12, workers=2)[0] == ev
+    assert B1.run_pool_shipped(P, 12, workers=2, n_gryz_tables=3)[0] == ev
+
+
 def test_electron_ids_above_2_32_have_streams_of_their_own(si10):
     """Electron ids are 48 bits: bits 32..47 go into Philox counter word 1
     beside the configuration id.  Electron 2^32 + k must not replay electron
