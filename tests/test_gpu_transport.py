@@ -220,6 +220,40 @@ def test_continuous_mode_matches_oracle(tilt, bethe):
     check_values(r.els, o, same)
 
 
+@pytest.mark.parametrize("bethe", ["classical", "JL", "explicit"])
+def test_continuous_step_rows_match_oracle_trace(bethe):
+    """Per-step rows of continuous mode (path, Bethe loss W = l * S(E_pre),
+    Rutherford angle, azimuth, energies, directions) for the first 12
+    electrons of Au 30 keV at 70 deg, from the FP64 and the FP32 kernels,
+    against the oracle's trace."""
+    from mcd import engine
+    tr = engine.Transport("Au", 30000.0, 1000.0, s_tilt=70.0, mode="cont", bethe_model=bethe)
+    rec = 12
+    _, tro, _ = O.run(tr.cfg, None, None, 5, 0, rec, trace_cap=400000)
+    for prec, tol in ((_abi.PREC_FP64, 1e-6), (_abi.PREC_FP32, 2e-5)):
+        s = tr.run(32, seed=5, scats=True, scat_max_electron=rec, precision=prec).scats
+        if prec == _abi.PREC_FP32:
+            # FP32 trajectories may part from the FP64 ones; compare the
+            # electrons whose step counts agree
+            keep = np.isin(s["electron"], [e for e in range(rec)
+                                           if (s["electron"] == e).sum() == (tro["electron"] == e).sum()])
+            assert keep.mean() >= 0.9, keep.mean()
+            s = s[keep]
+            t = tro[np.isin(tro["electron"], np.unique(s["electron"]))]
+        else:
+            t = tro
+        assert len(s) == len(t)
+        assert np.array_equal(s["electron"], t["electron"]) and np.array_equal(s["step"], t["step"])
+        assert np.array_equal(s["channel"], t["channel"])
+        assert np.array_equal(s["outcome"], t["outcome"].astype(np.uint8))
+        for k in ("E", "energy", "pathl", "az"):
+            assert np.allclose(s[k], t[k], rtol=tol, atol=0), (prec, k)
+        assert np.allclose(s["E_loss"], t["E_loss"], rtol=10 * tol, atol=1e-3), prec
+        assert np.allclose(s["pol"], t["pol"], rtol=10 * tol, atol=1e-6), prec
+        for k in ("dx", "dy", "dz"):
+            assert np.abs(s[k] - t[k]).max() < 20 * tol, (prec, k)
+
+
 def test_split_launches_equal_one_launch(cu20):
     """Philox keys use the global electron id: two half-range launches give
     bit-identical records to one launch (the sharding invariance a multi-GPU
