@@ -220,6 +220,31 @@ def test_continuous_mode_matches_oracle(tilt, bethe):
     check_values(r.els, o, same)
 
 
+@pytest.mark.parametrize("material,E0,Emin", [("Au", 30000.0, 1000.0), ("Al", 5000.0, 500.0)])
+def test_ds_other_materials_match_oracle(material, E0, Emin):
+    """DS parity at 11 channels (Au: Rutherford, Moller, Quinn and eight
+    Gryzinski shells in the pick) and for Al at 5 keV: the FP64 kernel equals
+    the oracle trajectory by trajectory, the FP32 kernel within the FP32
+    bar, and the per-step channel sequence of the first electrons matches."""
+    from mcd import engine
+    tr = engine.Transport(material, E0, Emin, Wc=100.0, s_tilt=70.0, tol_E=1e-2, tol_W=1e-2)
+    tm, tg = oracle_for(tr)
+    n = 2000
+    o, _, ev = O.run(tr.cfg, tm, tg, 3, 0, n, nthreads=16)
+    r64 = tr.run(n, seed=3, precision=_abi.PREC_FP64)
+    assert identical(r64.els, o).all() and r64.events == ev
+    check_values(r64.els, o, identical(r64.els, o), rel_E=1e-7, dir_abs=1e-7, pos_rel=1e-7)
+    r = tr.run(n, seed=3)
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999, same.mean()
+    check_values(r.els, o, same)
+    _, tro, _ = O.run(tr.cfg, tm, tg, 3, 0, 8, trace_cap=400000)
+    s = tr.run(8, seed=3, scats=True, precision=_abi.PREC_FP64).scats
+    assert len(s) == len(tro) and np.array_equal(s["channel"], tro["channel"])
+    assert np.array_equal(s["E_loss"], tro["E_loss"].astype(np.float32))
+    assert len(np.unique(s["channel"])) >= 4
+
+
 @pytest.mark.parametrize("bethe", ["classical", "JL", "explicit"])
 def test_continuous_step_rows_match_oracle_trace(bethe):
     """Per-step rows of continuous mode (path, Bethe loss W = l * S(E_pre),
