@@ -358,6 +358,26 @@ def test_max_scatt_and_thin_film_outcomes():
         assert tr.run(n, seed=6, guard=guard).els.tobytes() == r.els.tobytes(), guard
 
 
+def test_continuous_max_scatt_and_thin_film_outcomes():
+    """Continuous mode (Au 30 keV, JL) with maxScatt 150 and a 3000 A film:
+    'long', 'trsm' and 'bks' outcomes against the oracle, FP64 kernel exactly
+    and the FP32 kernel within its bar."""
+    from mcd import engine
+    tr = engine.Transport("Au", 30000.0, 1000.0, s_tilt=0.0, mode="cont", bethe_model="JL", thickness=3000.0,
+                          max_scatt=150)
+    n = 2000
+    o, _, ev = O.run(tr.cfg, None, None, 6, 0, n, nthreads=16)
+    for out in (_abi.OUT_LONG, _abi.OUT_TRSM, _abi.OUT_BKS):
+        assert (o["outcome"] == out).sum() > 200
+    r64 = tr.run(n, seed=6, precision=_abi.PREC_FP64)
+    assert identical(r64.els, o).all() and r64.events == ev
+    r = tr.run(n, seed=6)
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999, same.mean()
+    check_values(r.els, o, same)
+    assert int(r.counters[_abi.CNT_ELECTRONS]) == n
+
+
 def test_invalid_configuration_is_rejected_before_launch():
     from mcd import engine
     from mcd._lib import MCDError
