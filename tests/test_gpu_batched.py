@@ -114,3 +114,35 @@ def test_ragged_batch_with_empty_and_single_electron_items(small_batch, guard):
         empty.counters.zero_()
         empty.launch(seed=8, use_graph=use_graph, guard=guard, block=640 if guard == "service" else 0)
         assert all(not r.counters.any() for r in empty.results())
+
+
+def test_batch_with_thin_film_and_max_scatt_items():
+    """Items with a finite thickness and maxScatt (DS and continuous) in one
+    batch give the per-configuration records ('long' and 'trsm' outcomes
+    included) in every guard form."""
+    import os
+    from mcd import _abi, engine
+    old = os.environ.get("MCD_PICK_NODES")
+    os.environ["MCD_PICK_NODES"] = "128"
+    try:
+        trs = [engine.Transport("Si", 10000.0, 1000.0, Wc=100.0, s_tilt=0.0, tol_E=1e-2, tol_W=1e-2,
+                                thickness=3000.0, max_scatt=80),
+               engine.Transport("Au", 30000.0, 1000.0, s_tilt=0.0, mode="cont", bethe_model="JL", thickness=3000.0,
+                                max_scatt=150),
+               engine.Transport("Cu", 20000.0, 1000.0, Wc=100.0, s_tilt=70.0, tol_E=1e-2, tol_W=1e-2)]
+    finally:
+        if old is None:
+            del os.environ["MCD_PICK_NODES"]
+        else:
+            os.environ["MCD_PICK_NODES"] = old
+    sizes = [3000, 2000, 4000]
+    refs = [tr.run(n, seed=6, config_id=i) for i, (tr, n) in enumerate(zip(trs, sizes))]
+    for out in (_abi.OUT_LONG, _abi.OUT_TRSM):
+        assert (refs[0].els["outcome"] == out).sum() > 50 and (refs[1].els["outcome"] == out).sum() > 50
+    b = engine.Batch(list(zip(trs, sizes)), pick_nodes=128)
+    for guard in (None, "deferred", "service"):
+        b.counters.zero_()
+        b.launch(seed=6, use_graph=False, guard=guard, block=512 if guard == "service" else 0)
+        for i, (r, ref) in enumerate(zip(b.results(), refs)):
+            assert r.els.tobytes() == ref.els.tobytes(), (guard, i)
+            assert int(r.counters[_abi.CNT_EVENTS]) == ref.events, (guard, i)
