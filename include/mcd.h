@@ -180,7 +180,8 @@ enum {
   MCD_CNT_FP64_PICKS = 6, /* picks settled by the FP64 guard */
   MCD_CNT_SCAT_ROWS = 7,  /* scatter rows requested (may exceed capacity) */
   MCD_CNT_QUEUE = 8,      /* internal: the launch's work counter (mcd_forward resets it) */
-  MCD_CNT_GUARD_TIMEOUTS = 9, /* service guard: picks left unsettled (must be 0) */
+  MCD_CNT_GUARD_TIMEOUTS = 9, /* service guard: picks left unsettled (must be 0; otherwise the
+                                 launch's records are not valid) */
   MCD_NCOUNTERS = 16
 };
 
