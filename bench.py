@@ -322,6 +322,7 @@ def main():
         torch.cuda.synchronize(dev)
         del src, dst
     c = cnt.cpu().numpy().view("uint64")
+    engine.check_guard(c, "timed headline launches")  # records of a launch with unsettled picks are invalid
     events = int(c[_abi.CNT_EVENTS])
     value = events / wall
     ev_per_launch = events / a.steps

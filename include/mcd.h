@@ -13,11 +13,12 @@
  *
  * Device pointers passed in are plain HIP device addresses; streams are
  * hipStream_t passed as void* (NULL = the null stream).  One context is used
- * by one host thread at a time.  mcd_forward keeps a launch's work counter in
- * the caller's counters block (MCD_CNT_QUEUE), so inline or served launches
- * with separate counters blocks may run concurrently on different streams of
- * one context; deferred launches and mcd_backward use counters held by the
- * context, so such calls on one context must not overlap in time.
+ * by one host thread at a time.  mcd_forward keeps a launch's work counters in
+ * the caller's counters block (MCD_CNT_QUEUE, and MCD_CNT_PARKED /
+ * MCD_CNT_RESUMED for the deferred guard, whose park buffer the context keeps
+ * per stream), so launches with separate counters blocks may run concurrently
+ * on different streams of one context.  mcd_backward uses a work counter held
+ * by the context, so backward calls on one context must not overlap in time.
  */
 #ifndef MCD_H
 #define MCD_H
@@ -180,8 +181,12 @@ enum {
   MCD_CNT_FP64_PICKS = 6, /* picks settled by the FP64 guard */
   MCD_CNT_SCAT_ROWS = 7,  /* scatter rows requested (may exceed capacity) */
   MCD_CNT_QUEUE = 8,      /* internal: the launch's work counter (mcd_forward resets it) */
-  MCD_CNT_GUARD_TIMEOUTS = 9, /* service guard: picks left unsettled (must be 0; otherwise the
-                                 launch's records are not valid) */
+  MCD_CNT_GUARD_TIMEOUTS = 9, /* guard picks left unsettled: a served-guard wait that timed out, or
+                                 a parked electron the resume kernel could not place (must be 0;
+                                 otherwise the launch's records are not valid) */
+  MCD_CNT_PARKED = 10,    /* internal, deferred guard: electrons parked by the first kernel
+                             (mcd_forward resets it) */
+  MCD_CNT_RESUMED = 11,   /* internal, deferred guard: the resume kernel's work counter */
   MCD_NCOUNTERS = 16
 };
 
@@ -248,6 +253,13 @@ int mcd_tables_destroy(mcd_tables* t);
 /* Channel-pick table diagnostics (no reference counterpart): node count, cells
  * with a cross-section crossing, cells always settled in FP64, guard band. */
 int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap_cells, int* n_exact_cells, float* guard_delta);
+/* Rebuild the DS channel-pick table with n_nodes nodes (16..8192; tables are
+ * built with 512).  No reference counterpart: the pick table is this build's
+ * interpolation of the per-step sigma set (scattering.py:440-527).  The node
+ * count changes FP32 rounding of the mean free path, not the picks; a
+ * per-configuration launch reproduces a batch item's records when both use
+ * the same count.  Waits for the device before freeing the old table. */
+int mcd_tables_set_pick_nodes(mcd_ctx* ctx, mcd_tables* t, int n_nodes);
 
 /* Run electrons [first, first+n) of one configuration.  els: device array of
  * n records (index = electron - first).  scats may be NULL.  counters: device

@@ -15,12 +15,20 @@
 
 using namespace mcd;
 
+// Park buffer of the deferred guard for launches on one stream: launches on
+// one stream run one after the other, so they can share it; launches on
+// different streams get different buffers and may overlap.
+struct ParkBuf {
+  hipStream_t stream;
+  ParkRec* p;
+  uint64_t cap;  // electrons
+};
+
 struct mcd_ctx {
   int device;
   int n_cu;
-  unsigned long long* d_queue;  // [0] backward work counter, [1] parked electrons, [2] resume counter
-  ParkRec* d_park;              // deferred-guard park buffer
-  uint64_t park_cap;            // its capacity in electrons
+  unsigned long long* d_queue;  // [0] backward work counter
+  std::vector<ParkBuf> parks;   // deferred-guard park buffers, one per stream
   char err[512];
 };
 
@@ -92,8 +100,6 @@ extern "C" int mcd_ctx_create(int device, mcd_ctx** out) {
   mcd_ctx* c = new mcd_ctx();
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
-  c->d_park = nullptr;
-  c->park_cap = 0;
   c->err[0] = 0;
   if (hipMalloc(&c->d_queue, 64 * sizeof(unsigned long long)) != hipSuccess) {
     delete c;
@@ -107,7 +113,8 @@ extern "C" int mcd_ctx_destroy(mcd_ctx* ctx) {
   if (!ctx) return MCD_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipFree(ctx->d_queue);
-  if (ctx->d_park) (void)hipFree(ctx->d_park);
+  for (ParkBuf& b : ctx->parks)
+    if (b.p) (void)hipFree(b.p);
   delete ctx;
   return MCD_OK;
 }
@@ -180,7 +187,8 @@ static DevCfg make_devcfg(const mcd_config& cfg, const mcd_tables* t, uint32_t c
   c.bethe_k1_f = (float)(2. * kPiEFour * m.atnd * (double)m.Z * 1e-14);
   c.bethe_c1_f = (float)(1.166 / cfg.J);
   c.bethe_kJ_f = (float)(m.bethe_k * cfg.J);
-  c.expl_k_f = (float)(2. * kPiEFour * m.atnd * 1e-14 * cfg.explicit_Zeff);
+  c.expl_k = 2. * kPiEFour * m.atnd * 1e-14 * cfg.explicit_Zeff;
+  c.expl_k_f = (float)c.expl_k;
   c.expl_inv_Estar_f = (float)exp(-cfg.explicit_lnEstar);
   for (int i = 0; i < 3; ++i) c.dir0_f[i] = (float)c.dir0[i];
   if (t && t->has_w) {
@@ -470,6 +478,20 @@ extern "C" int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap, i
   return MCD_OK;
 }
 
+extern "C" int mcd_tables_set_pick_nodes(mcd_ctx* ctx, mcd_tables* t, int n_nodes) {
+  if (!ctx || !t) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_set_pick_nodes: NULL argument");
+  if (n_nodes < 16 || n_nodes > 8192)
+    return set_err(ctx, MCD_ERR_INVALID, "pick-table node count %d outside [16, 8192]", n_nodes);
+  if (t->cfg.mode != MCD_MODE_DS || !t->has_w)
+    return set_err(ctx, MCD_ERR_INVALID, "continuous-mode tables have no channel-pick table");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipDeviceSynchronize());  // launches may still read the old table
+  int rc = build_pick_table(ctx, t, n_nodes);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipDeviceSynchronize());
+  return MCD_OK;
+}
+
 // occupancy variant used when the caller does not choose one
 // Occupancy of the one-kernel FP32 transport: 4 resident wavefronts per SIMD
 // (128 VGPRs, no spills) hides more of the table and LDS latency than the
@@ -522,8 +544,8 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   R.els = els;
   R.counters = (unsigned long long*)counters;
   // the launch's work counter is slot MCD_CNT_QUEUE of the caller's counters
-  // block, so launches with separate blocks share no device state (the
-  // deferred form still uses the context's park buffer and counts)
+  // block, so launches with separate blocks share no device counters (the
+  // deferred form's park buffer is per stream)
   unsigned long long* q = (unsigned long long*)counters + MCD_CNT_QUEUE;
   R.queue = q;
   if (scats && scats->rows && scats->capacity) {
@@ -568,26 +590,35 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     HIP_TRY(ctx, launch_fill_u64(q, 0ull, 1, s));
     HIP_TRY(ctx, launch_transport_svc(c, R, minw, (int)grid, block, s));
   } else if (deferred) {
-    if (ctx->park_cap < n) {
-      // the park buffer is reused across calls: wait for work that may still read it
-      HIP_TRY(ctx, hipDeviceSynchronize());
-      if (ctx->d_park) { (void)hipFree(ctx->d_park); ctx->d_park = nullptr; ctx->park_cap = 0; }
-      if (hipMalloc(&ctx->d_park, n * sizeof(ParkRec)) != hipSuccess) {
-        ctx->d_park = nullptr;
+    // the stream's park buffer (launches on other streams use their own)
+    ParkBuf* pb = nullptr;
+    for (ParkBuf& b : ctx->parks)
+      if (b.stream == s) pb = &b;
+    if (!pb) {
+      ctx->parks.push_back(ParkBuf{s, nullptr, 0});
+      pb = &ctx->parks.back();
+    }
+    if (pb->cap < n) {
+      // earlier launches on this stream may still read the buffer
+      HIP_TRY(ctx, hipStreamSynchronize(s));
+      if (pb->p) { (void)hipFree(pb->p); pb->p = nullptr; pb->cap = 0; }
+      if (hipMalloc(&pb->p, n * sizeof(ParkRec)) != hipSuccess) {
+        pb->p = nullptr;
         return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc of the park buffer (%llu electrons) failed",
                        (unsigned long long)n);
       }
-      ctx->park_cap = n;
+      pb->cap = n;
     }
-    R.park = ctx->d_park;
-    R.park_count = ctx->d_queue + 1;
-    R.resume_queue = ctx->d_queue + 2;
+    R.park = pb->p;
+    // parked count and resume counter: slots of the caller's counters block
+    R.park_count = (unsigned long long*)counters + MCD_CNT_PARKED;
+    R.resume_queue = (unsigned long long*)counters + MCD_CNT_RESUMED;
     uint64_t g1 = (uint64_t)ctx->n_cu * (uint64_t)deferred_blocks_per_cu(c, 1, block);
     uint64_t g2 = (uint64_t)ctx->n_cu * (uint64_t)deferred_blocks_per_cu(c, 2, block);
     if (g1 > want) g1 = want;
     if (g2 > want) g2 = want;
     HIP_TRY(ctx, launch_fill_u64(q, 0ull, 1, s));
-    HIP_TRY(ctx, launch_fill_u64(ctx->d_queue + 1, 0ull, 2, s));
+    HIP_TRY(ctx, launch_fill_u64(R.park_count, 0ull, 2, s));
     HIP_TRY(ctx, launch_transport_deferred(c, R, 1, (int)(g1 < 1 ? 1 : g1), block, s));
     HIP_TRY(ctx, launch_transport_deferred(c, R, 2, (int)(g2 < 1 ? 1 : g2), block, s));
   } else {

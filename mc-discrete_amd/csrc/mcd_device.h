@@ -79,6 +79,7 @@ struct DevCfg {
   // FP64 copies (FP64 transport and the exact pick)
   double E0, Emin, Wc, Ef, Epl, atnd, thickness, c_alpha, ZZ, n_val;
   double J, bethe_kk, zeff, lnEstar, E_val;
+  double expl_k;  // explicit Bethe law: 2 pi e^4 n Z_eff 1e-14 (S = expl_k (ln E - lnEstar) / E)
   double Eb[MCD_MAX_SHELLS], ns[MCD_MAX_SHELLS];
   double dir0[3];
   // FP32 constants

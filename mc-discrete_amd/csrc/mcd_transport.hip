@@ -165,11 +165,11 @@ __device__ __forceinline__ void guard_perm(const PickEval<NCH>& pe, uint32_t& pl
 // a wavefront exits only after its last reply, so none is left waiting).
 // A posting wavefront sets its mailbox's bit in *pending, so an idle poll is
 // one LDS read between sleeps of 512 clocks (the idle service wavefront takes
-// few issue slots from the transport wavefronts on its SIMD).  It also stops
-// after kServiceIdleCap idle polls (~4 minutes), so it ends even if the
-// finished count were never reached; a transport wavefront left waiting then
-// flags its step (kGuardSpinCap).
-constexpr uint32_t kServiceIdleCap = 1u << 30;
+// few issue slots from the transport wavefronts on its SIMD).  The loop has
+// no poll limit of its own, so a launch may run as long as its work needs:
+// it ends because every transport wavefront ends (the work counter is finite,
+// every trajectory is capped at kMaxStepsOff steps and every wait for a reply
+// is bounded by kGuardSpinCap), and each of them adds to *done on exit.
 
 template <int NCH>
 __device__ void guard_service(const DevCfg& c, GuardBox* boxes, uint32_t* pending, uint32_t* done, int n_boxes) {
@@ -177,12 +177,10 @@ __device__ void guard_service(const DevCfg& c, GuardBox* boxes, uint32_t* pendin
   const uint32_t nb = (uint32_t)__builtin_amdgcn_readfirstlane(n_boxes);
   // a waiting transport wavefront is blocked until the reply: serve first
   __builtin_amdgcn_s_setprio(3);
-  uint32_t idle = 0;
-  while (idle < kServiceIdleCap) {
+  while (true) {
     if (__builtin_amdgcn_readfirstlane(box_load(pending)) == 0u) {
       if (__builtin_amdgcn_readfirstlane(box_load(done)) == nb) break;
       __builtin_amdgcn_s_sleep(8);
-      ++idle;
       continue;
     }
     // all 64 lanes are active here: lane 0 takes the pending set for the wavefront
@@ -217,7 +215,8 @@ __device__ void guard_service(const DevCfg& c, GuardBox* boxes, uint32_t* pendin
 // same records as phase 0: a parked step is redone from the same state and
 // Philox counter.  PHASE 3 (DS): guard-band picks go to the workgroup's
 // service wavefront through `box`; same records as phase 0.
-template <int MODE, int NCH, int PHASE = 0, bool PERIOD = false>
+// EXPL (continuous mode): the explicit Bethe law, evaluated in FP64.
+template <int MODE, int NCH, int PHASE = 0, bool PERIOD = false, bool EXPL = false>
 __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds,
                                                const DevTable* __restrict__ tabs, const RunArgs& R,
                                                uint32_t& n_events, uint32_t& n_fp64, GuardBox* box = nullptr,
@@ -253,12 +252,19 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
             if constexpr (PHASE == 2) {
               const ParkRec& q = R.park[idx];
               e = q.e;
-              ctr1 = c.config_id | (uint32_t)(e >> 32) << 16;
-              step = q.step;
-              flags = q.flags;
-              E.hi = q.Ehi; E.lo = q.Elo;
+              if (e - R.first >= R.n) {
+                // not an electron of this launch: its record cannot be placed,
+                // so the launch is reported invalid instead of writing out of range
+                atomicAdd(R.counters + MCD_CNT_GUARD_TIMEOUTS, 1ull);
+                active = false;
+              } else {
+                ctr1 = c.config_id | (uint32_t)(e >> 32) << 16;
+                step = q.step;
+                flags = q.flags;
+                E.hi = q.Ehi; E.lo = q.Elo;
 #pragma unroll
-              for (int j = 0; j < 3; ++j) { p[j] = q.p[j]; pc[j] = q.pc[j]; d[j] = q.d[j]; y[j] = q.y[j]; }
+                for (int j = 0; j < 3; ++j) { p[j] = q.p[j]; pc[j] = q.pc[j]; d[j] = q.d[j]; y[j] = q.y[j]; }
+              }
             } else {
               e = R.first + idx;
               ctr1 = c.config_id | (uint32_t)(e >> 32) << 16;
@@ -290,7 +296,8 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
 
     // ---- one loop iteration of trajectory_DS / trajectory_cont_cl ----
     if ((int64_t)step >= c.max_scatt) {  // num_scatt > maxScatt -> 'long'
-      write_rec(&R.els[e - R.first], E.hi, d, p, step, MCD_OUT_LONG, 0, flags);
+      const float sp[3] = {p[0] - pc[0], p[1] - pc[1], p[2] - pc[2]};  // compensated, as for other outcomes
+      write_rec(&R.els[e - R.first], E.hi, d, sp, step, MCD_OUT_LONG, 0, flags);
       active = false;
       continue;
     }
@@ -406,12 +413,21 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       l = -mfp * ln_f(u0);
       if (!(l < 1e4f)) sflags |= MCD_FLAG_LONG_PATH;
       // Bethe stopping power at the pre-step energy (stoppingPowers.py:112-215)
-      float S;
-      const float iE = __builtin_amdgcn_rcpf(E.hi);
-      if (c.bethe == MCD_BETHE_CLASSICAL) S = c.bethe_k1_f * iE * ln_f(c.bethe_c1_f * E.hi);
-      else if (c.bethe == MCD_BETHE_JL) S = c.bethe_k1_f * iE * ln_f(c.bethe_c1_f * (E.hi + c.bethe_kJ_f));
-      else S = c.expl_k_f * iE * ln_f(E.hi * c.expl_inv_Estar_f);
-      Wh = l * S;
+      if constexpr (EXPL) {
+        // explicit law (:177-215), its shell sum folded to Z_eff (ln E - ln E*),
+        // in FP64: near Emin ln E and ln E* are close, and their FP32
+        // difference would lose most of its bits; the loss l*S is kept as a pair
+        const double Ed = (double)E.hi + (double)E.lo;
+        const double Wd = (double)l * (c.expl_k * (log(Ed) - c.lnEstar) / Ed);
+        Wh = (float)Wd;
+        Wl = (float)(Wd - (double)Wh);
+      } else {
+        float S;
+        const float iE = __builtin_amdgcn_rcpf(E.hi);
+        if (c.bethe == MCD_BETHE_CLASSICAL) S = c.bethe_k1_f * iE * ln_f(c.bethe_c1_f * E.hi);
+        else S = c.bethe_k1_f * iE * ln_f(c.bethe_c1_f * (E.hi + c.bethe_kJ_f));
+        Wh = l * S;
+      }
       if (!f_lt_f2(Wh, E)) sflags |= MCD_FLAG_LOSS_GE_E;
       const float om = 1.f - u1;
       const float inv = __builtin_amdgcn_rcpf(om + a);
@@ -465,7 +481,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
 
 // MINW: minimum resident wavefronts per SIMD the register allocation must
 // allow (1 = compiler's choice, the default; others per launch).
-template <int MODE, int NCH, int MINW, int PHASE = 0>
+template <int MODE, int NCH, int MINW, int PHASE = 0, bool EXPL = false>
 __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R) {
   extern __shared__ float lds[];
   DevTable* tabs = nullptr;
@@ -474,7 +490,7 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
     __syncthreads();
   }
   uint32_t n_events = 0, n_fp64 = 0;
-  transport_loop<MODE, NCH, PHASE, true>(c, lds, tabs, R, n_events, n_fp64);
+  transport_loop<MODE, NCH, PHASE, true, EXPL>(c, lds, tabs, R, n_events, n_fp64);
   const int lane = threadIdx.x & 63;
   const uint64_t cnt[2] = {n_events, n_fp64};
   wave_add_counters(R.counters + MCD_CNT_EVENTS, cnt, 1, lane);
@@ -572,13 +588,13 @@ constexpr int kCfgFloats = (int)((sizeof(DevCfg) + 15) / 16 * 4);
 
 int batched_cfg_floats() { return kCfgFloats; }
 
-template <int MODE, int NCH, int PHASE>
+template <int MODE, int NCH, int PHASE, bool EXPL = false>
 __device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, const float* pick, uint64_t seed,
                                          uint32_t& ne, uint32_t& nf, GuardBox* box, uint32_t* pending) {
   RunArgs R = itR;
   R.seed = seed;
   // c is the wavefront's LDS copy, so its two table descriptors are in LDS too
-  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0>(c, pick, &c.tm, R, ne, nf, box, pending);
+  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0, false, EXPL>(c, pick, &c.tm, R, ne, nf, box, pending);
 }
 
 // The batched kernel's service wavefront: as guard_service, for mailboxes
@@ -590,12 +606,10 @@ __device__ void guard_service_batched(const float* lds, int slice_floats, GuardB
   const int lane = threadIdx.x & 63;
   const uint32_t nb = (uint32_t)__builtin_amdgcn_readfirstlane(n_boxes);
   __builtin_amdgcn_s_setprio(3);
-  uint32_t idle = 0;
-  while (idle < kServiceIdleCap) {
+  while (true) {  // ends as guard_service does: every transport wavefront ends and adds to *done
     if (__builtin_amdgcn_readfirstlane(box_load(pending)) == 0u) {
       if (__builtin_amdgcn_readfirstlane(box_load(done)) == nb) break;
       __builtin_amdgcn_s_sleep(8);
-      ++idle;
       continue;
     }
     uint32_t q = 0;
@@ -707,7 +721,12 @@ __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ?
     const DevCfg& c = *reinterpret_cast<const DevCfg*>(my);
     uint32_t ne = 0, nf = 0;
     if (mode == MCD_MODE_CONT) {
-      if (PHASE != 2) run_item<MCD_MODE_CONT, 4, PHASE>(c, it.R, pick, seed, ne, nf, nullptr, nullptr);
+      if (PHASE != 2) {
+        if (c.bethe == MCD_BETHE_EXPLICIT)
+          run_item<MCD_MODE_CONT, 4, PHASE, true>(c, it.R, pick, seed, ne, nf, nullptr, nullptr);
+        else
+          run_item<MCD_MODE_CONT, 4, PHASE>(c, it.R, pick, seed, ne, nf, nullptr, nullptr);
+      }
     } else {
       switch (nch) {
         case 4: run_item<MCD_MODE_DS, 4, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
@@ -955,29 +974,30 @@ hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsign
 // ---------------------------------------------------------------------------
 // host-side launchers
 
-template <int MODE, int NCH, int MINW>
+template <int MODE, int NCH, int MINW, bool EXPL>
 static hipError_t launch_f32(const DevCfg& c, const RunArgs& R, int grid, int block, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_transport_f32<MODE, NCH, MINW>), dim3(grid), dim3(block), lds, s, c, R);
+  hipLaunchKernelGGL((k_transport_f32<MODE, NCH, MINW, 0, EXPL>), dim3(grid), dim3(block), lds, s, c, R);
   return hipGetLastError();
 }
 
-template <int MODE, int NCH, int MINW>
+template <int MODE, int NCH, int MINW, bool EXPL>
 static int occ_f32(int block, size_t lds) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MODE, NCH, MINW>, block, lds) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_f32<MODE, NCH, MINW, 0, EXPL>, block, lds) !=
+      hipSuccess)
     return 1;
   return nb < 1 ? 1 : nb;
 }
 
 // dispatch over (channel count, occupancy variant)
-template <int MODE, int NCH>
+template <int MODE, int NCH, bool EXPL = false>
 static hipError_t dispatch_w(int minw, const DevCfg& c, const RunArgs* R, int grid, int block, size_t lds,
                              hipStream_t s, int* occ) {
   switch (minw) {
 #define MCD_W(W)                                                   \
   case W:                                                          \
-    if (occ) { *occ = occ_f32<MODE, NCH, W>(block, lds); return hipSuccess; } \
-    return launch_f32<MODE, NCH, W>(c, *R, grid, block, lds, s);
+    if (occ) { *occ = occ_f32<MODE, NCH, W, EXPL>(block, lds); return hipSuccess; } \
+    return launch_f32<MODE, NCH, W, EXPL>(c, *R, grid, block, lds, s);
     MCD_W(1) MCD_W(4) MCD_W(5) MCD_W(6) MCD_W(8)
 #undef MCD_W
     default: return hipErrorInvalidValue;
@@ -986,7 +1006,10 @@ static hipError_t dispatch_w(int minw, const DevCfg& c, const RunArgs* R, int gr
 
 static hipError_t dispatch_f32(int minw, const DevCfg& c, const RunArgs* R, int grid, int block, hipStream_t s,
                                int* occ) {
-  if (c.mode == MCD_MODE_CONT) return dispatch_w<MCD_MODE_CONT, 4>(minw, c, R, grid, block, 0, s, occ);
+  if (c.mode == MCD_MODE_CONT) {
+    if (c.bethe == MCD_BETHE_EXPLICIT) return dispatch_w<MCD_MODE_CONT, 4, true>(minw, c, R, grid, block, 0, s, occ);
+    return dispatch_w<MCD_MODE_CONT, 4>(minw, c, R, grid, block, 0, s, occ);
+  }
   const size_t lds = single_lds_bytes(c);
   switch (c.n_shells + 3) {
     case 4: return dispatch_w<MCD_MODE_DS, 4>(minw, c, R, grid, block, lds, s, occ);

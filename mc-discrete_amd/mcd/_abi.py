@@ -19,7 +19,7 @@ TABLE_MOLLER, TABLE_GRYZ = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 
 (CNT_EVENTS, CNT_ELECTRONS, CNT_BKS, CNT_ABS, CNT_TRSM, CNT_LONG, CNT_FP64_PICKS, CNT_SCAT_ROWS, CNT_QUEUE,
- CNT_GUARD_TIMEOUTS) = range(10)
+ CNT_GUARD_TIMEOUTS, CNT_PARKED, CNT_RESUMED) = range(12)
 NCOUNTERS = 16
 GRAD_WC, GRAD_TILT, GRAD_J, GRAD_LNRHO = range(4)
 NGRAD = 4

@@ -54,6 +54,8 @@ def lib():
     L.mcd_tables_destroy.argtypes = [vp]
     L.mcd_tables_info.restype = i32
     L.mcd_tables_info.argtypes = [vp, P(i32), P(i32), P(i32), P(C.c_float)]
+    L.mcd_tables_set_pick_nodes.restype = i32
+    L.mcd_tables_set_pick_nodes.argtypes = [vp, vp, i32]
     L.mcd_forward.restype = i32
     L.mcd_forward.argtypes = [vp, P(_abi.Config), vp, u64, u32, u64, u64, vp, P(_abi.ScatterBuf), vp,
                               P(_abi.Launch), vp]

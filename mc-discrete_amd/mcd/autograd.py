@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .engine import Transport, backscatter_stats
+from .engine import Transport, backscatter_stats, check_guard
 from .material import Material, bethe_J
 
 
@@ -86,6 +86,7 @@ class BackscatterSpec:
             total += c
             hist += h
             done += k
+        check_guard(cnt.cpu().numpy().view(np.uint64), "forward_stats")  # one check over every chunk
         return total, hist
 
 

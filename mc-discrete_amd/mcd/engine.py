@@ -44,6 +44,21 @@ class RunResult:
         return int(self.counters[_abi.CNT_EVENTS])
 
 
+class GuardError(RuntimeError):
+    """A launch left guard picks unsettled (counter CNT_GUARD_TIMEOUTS > 0):
+    its records are not valid and must not be used."""
+
+
+def check_guard(counters, what="launch"):
+    """Raise GuardError when a counters block (uint64[NCOUNTERS], host or
+    device) reports unsettled guard picks."""
+    v = counters[_abi.CNT_GUARD_TIMEOUTS]
+    v = int(v.item()) if hasattr(v, "item") else int(v)
+    if v:
+        raise GuardError(f"{what}: {v} guard picks were not settled (no reply from the service wavefront "
+                         "within the wait bound, or a parked electron outside the launch); records are invalid")
+
+
 class ScatterOverflow(RuntimeError):
     """The scatter rows of a run exceeded the row buffer (no rows are lost on
     the device side beyond the capacity being full: rerunning the same
@@ -81,7 +96,8 @@ class Transport:
     def __init__(self, material: str | Material, E0: float, Emin: float, Wc: float = 50.0, s_tilt: float = 0.0,
                  mode: str = "DS", bethe_model: str = "classical", tol_E: float = 1e-3, tol_W: float = 1e-3,
                  thickness: float = float("inf"), max_scatt: int = 0, J: float | None = None,
-                 table_precision: int = _abi.PREC_FP64, device: int = 0, grids=None, cdfs=None):
+                 table_precision: int = _abi.PREC_FP64, device: int = 0, grids=None, cdfs=None,
+                 pick_nodes: int = 0):
         self.mat = material if isinstance(material, Material) else Material(material)
         self.device = device
         self.cfg = _abi.make_config(self.mat, E0, Emin, Wc, s_tilt, thickness, mode, bethe_model, max_scatt, J)
@@ -107,6 +123,9 @@ class Transport:
                 cg = np.ascontiguousarray(cdfs[1], dtype=np.float32)
                 check(lib().mcd_tables_upload(ctx, C.byref(self.cfg), C.byref(gm), cm.ctypes.data, C.byref(gg),
                                               cg.ctypes.data, C.byref(self._tables)), ctx)
+            if pick_nodes:
+                # channel-pick table of a given node count (a batch item's, to reproduce its records)
+                check(lib().mcd_tables_set_pick_nodes(ctx, self._tables, int(pick_nodes)), ctx)
         else:
             check(lib().mcd_tables_create(ctx, C.byref(self.cfg), None, None, int(table_precision),
                                           C.byref(self._tables)), ctx)
@@ -190,9 +209,7 @@ class Transport:
                     waves_per_simd=waves_per_simd, guard=guard, refill_period=refill_period)
         torch.cuda.synchronize(dev)
         counters = cnt.cpu().numpy().view(np.uint64)
-        if counters[_abi.CNT_GUARD_TIMEOUTS]:
-            raise RuntimeError(f"{int(counters[_abi.CNT_GUARD_TIMEOUTS])} guard picks were not settled "
-                               "(service wavefront did not reply within the wait bound)")
+        check_guard(counters)
         e = els.cpu().numpy()[: n * _abi.ELECTRON_DTYPE.itemsize].view(_abi.ELECTRON_DTYPE)
         srows = None
         if rows is not None:
@@ -205,14 +222,16 @@ class Transport:
 
 
     # -- differentiation ------------------------------------------------------
-    def run_chunks(self, n, chunk, seed=1, first=0, streams=2):
+    def run_chunks(self, n, chunk, seed=1, first=0, streams=2, guard=None):
         """Electrons [first, first+n) in launches of `chunk` electrons, yielded
         in order as (first electron, RunResult without scatter rows).  The
         launches alternate between `streams` HIP streams (default priority),
         each with its own record and counter buffers, so the device starts the
         next launch while the previous one's last trajectories finish and while
         the host handles the previous results.  Records equal those of one
-        launch over the whole range (they are keyed by electron id)."""
+        launch over the whole range (they are keyed by electron id).  guard:
+        as for launch (every form may overlap on separate streams: the
+        deferred form's park buffer is per stream)."""
         dev = torch.device("cuda", self.device)
         esz = _abi.ELECTRON_DTYPE.itemsize
         chunk = max(1, int(chunk))
@@ -226,7 +245,7 @@ class Transport:
             m = min(chunk, int(first) + int(n) - starts[k])
             with torch.cuda.stream(s):
                 cnt.zero_()
-                self.launch(m, els, cnt, seed=seed, first=starts[k], stream=s)
+                self.launch(m, els, cnt, seed=seed, first=starts[k], stream=s, guard=guard)
             return k, m
 
         pending = [launch(k) for k in range(min(len(ss), len(starts)))]
@@ -240,8 +259,7 @@ class Transport:
             if nxt < len(starts):
                 pending.append(launch(nxt))
                 nxt += 1
-            if counters[_abi.CNT_GUARD_TIMEOUTS]:
-                raise RuntimeError(f"{int(counters[_abi.CNT_GUARD_TIMEOUTS])} guard picks were not settled")
+            check_guard(counters)
             yield starts[k], RunResult(els=e, counters=counters, scats=None)
 
     def _bwd(self, n, seed, first, config_id, grad_out, out_vjp, out_jac, n_bins, rel_step, graze, block,
@@ -337,7 +355,8 @@ class Batch:
         torch.cuda.synchronize(self.device)
         cnt = self.counters.cpu().numpy().view(np.uint64).reshape(len(self.items), _abi.NCOUNTERS)
         out = []
-        for (tr, n), e, c in zip(self.items, self.els, cnt):
+        for i, ((tr, n), e, c) in enumerate(zip(self.items, self.els, cnt)):
+            check_guard(c, f"batch item {i}")
             rec = e.cpu().numpy()[: n * _abi.ELECTRON_DTYPE.itemsize].view(_abi.ELECTRON_DTYPE)
             out.append(RunResult(els=rec, counters=c.copy(), scats=None))
         return out
@@ -349,4 +368,5 @@ def grids_for(mat: Material, E0, Emin, Wc, tol_E, tol_W):
             gryz_grid(mat, E0, Emin).build_grids(tol_E, tol_W))
 
 
-__all__ = ["Transport", "Batch", "RunResult", "context", "grids_for", "TableGrid", "backscatter_stats"]
+__all__ = ["Transport", "Batch", "RunResult", "context", "grids_for", "TableGrid", "backscatter_stats",
+           "GuardError", "check_guard"]

@@ -27,9 +27,13 @@ def batched_configs(seed=3, scale=1.0):
     return out
 
 
-def build_transports(configs, device=0):
+def build_transports(configs, device=0, pick_nodes=0):
+    """One Transport per configuration; pick_nodes > 0 sets the node count of
+    the DS channel-pick tables (the count a batch's items use, so that
+    per-configuration launches reproduce the batch's records)."""
     from .engine import Transport
 
     return [Transport(c["material"], c["E0"], c["Emin"], Wc=c["Wc"], s_tilt=c["s_tilt"], mode=c["mode"],
-                      bethe_model=c["bethe_model"], tol_E=c["tol"], tol_W=c["tol"], device=device)
+                      bethe_model=c["bethe_model"], tol_E=c["tol"], tol_W=c["tol"], device=device,
+                      pick_nodes=pick_nodes if c["mode"] == "DS" else 0)
             for c in configs]

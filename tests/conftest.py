@@ -61,3 +61,25 @@ def gpu_available():
 @pytest.fixture(scope="session")
 def rng():
     return np.random.default_rng(12345)
+
+
+def identical(g, o):
+    """Trajectories with the same outcomes and scatter count (records of the
+    HIP kernels against oracle records)."""
+    return (g["outcome"] == o["outcome"]) & (g["outcome2"] == o["outcome2"]) & (g["n_scatt"] == o["n_scatt"])
+
+
+def check_values(g, o, same, rel_E=1e-5, dir_abs=1e-5, pos_rel=1e-5):
+    """Energies and directions of identical trajectories within tolerance; the
+    position error is measured against the trajectory's length scale
+    (mean free path x events), since a coordinate can sit near zero."""
+    if not np.any(same):
+        return 0.0, 0.0, 0.0
+    E_err = np.abs(g["final_E"][same] - o["final_E"][same]) / o["final_E"][same]
+    assert E_err.max() <= rel_E, E_err.max()
+    d_err = np.abs(g["dir"][same] - o["dir"][same]).max()
+    assert d_err <= dir_abs, d_err
+    scale = np.maximum(np.linalg.norm(o["pos"][same], axis=1), 40.0 * np.sqrt(o["n_scatt"][same] + 1.0))
+    p_err = (np.linalg.norm(g["pos"][same] - o["pos"][same], axis=1) / scale).max()
+    assert p_err <= pos_rel, p_err
+    return E_err.max(), d_err, p_err

@@ -1,28 +1,90 @@
-"""Batched multi-configuration launch against per-configuration launches of
-the same Philox streams (item i uses counter word config = i)."""
+"""The 64-configuration batched workload: every configuration against the
+FP64 oracle (same Philox streams, item i uses counter word config = i), the
+batched launch against per-configuration launches, and the full-size batch
+through properties that do not depend on its size."""
+import math
+
 import numpy as np
 import pytest
+
+import oracle as O
+from conftest import check_values, identical
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
 def small_batch():
-    """Per-configuration transports whose pick tables have the node count the
-    batch uses (given explicitly), so both launch paths read identical tables."""
-    import os
+    """The 64 configurations at 1/64 of their electron counts (2^8 .. 2^14
+    each), with per-configuration transports whose pick tables have the node
+    count the batch uses, so both launch paths read identical tables."""
     from mcd import workloads
-    old = os.environ.get("MCD_PICK_NODES")
-    os.environ["MCD_PICK_NODES"] = "128"
-    try:
-        cfgs = workloads.batched_configs(scale=1 / 64)  # 2^8 .. 2^14 electrons per configuration
-        trs = workloads.build_transports(cfgs)
-    finally:
-        if old is None:
-            del os.environ["MCD_PICK_NODES"]
-        else:
-            os.environ["MCD_PICK_NODES"] = old
+    cfgs = workloads.batched_configs(scale=1 / 64)
+    trs = workloads.build_transports(cfgs, pick_nodes=128)
     return cfgs, trs
+
+
+def oracle_tables(tr):
+    """Oracle tables holding the device-built CDF bytes (DS), else None."""
+    if tr.grids is None or tr.cfg.mode != 0:
+        return None, None
+    gm, gg = tr.grids
+    return O.Table(gm.Es, gm.Ws, tr.cdf(0)), O.Table(gg.Es, gg.Ws, tr.cdf(1))
+
+
+@pytest.fixture(scope="module")
+def batch_oracle(small_batch):
+    """FP64 oracle records of every item of the 1/64-scale batch (seed 3,
+    counter word config = item index)."""
+    cfgs, trs = small_batch
+    out = []
+    for i, (tr, c) in enumerate(zip(trs, cfgs)):
+        tm, tg = oracle_tables(tr)
+        o, _, ev = O.run(tr.cfg, tm, tg, 3, 0, c["n"], config_id=i, nthreads=16)
+        out.append((o, ev))
+    return out
+
+
+def _name(c):
+    return f"{c['material']} {c['E0'] / 1000:g} keV {c['s_tilt']:g} deg {c['mode']}"
+
+
+def test_every_batch_configuration_fp64_kernel_equals_oracle(small_batch, batch_oracle):
+    """All 64 configurations (Emin 0.5 keV, Wc 100 eV, tol 1e-3; DS items with
+    6, 7 and 11 channels, the Cu 5 keV and Au 5 and 10 keV DS items whose inner
+    shells lie above E0, continuous Joy-Luo items): the all-FP64 kernel gives
+    the oracle's records trajectory by trajectory and the same event count
+    (multiScatter.py:66-116, singleScatter.py:10-41,205-239)."""
+    from mcd import _abi
+    cfgs, trs = small_batch
+    for i, (tr, c, (o, ev)) in enumerate(zip(trs, cfgs, batch_oracle)):
+        print(f"item {i}: {_name(c)}, {c['n']} electrons", flush=True)
+        r64 = tr.run(c["n"], seed=3, config_id=i, precision=_abi.PREC_FP64)
+        same = identical(r64.els, o)
+        assert same.all(), (i, _name(c), np.flatnonzero(~same)[:5])
+        assert r64.events == ev, (i, _name(c))
+        check_values(r64.els, o, same, rel_E=1e-7, dir_abs=1e-7, pos_rel=1e-7)
+
+
+def test_every_batch_configuration_fp32_batched_launch_matches_oracle(small_batch, batch_oracle):
+    """One batched launch of all 64 configurations (FP32) against the oracle:
+    at least 99.9 % of all trajectories identical end to end and no item
+    with more than max(1, 0.1 %) differing ones; energies, directions and
+    positions of identical trajectories within the FP32 bar."""
+    from mcd import engine
+    cfgs, trs = small_batch
+    b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs, cfgs)], pick_nodes=128)
+    b.launch(seed=3, use_graph=True)
+    n_same = n_all = 0
+    for i, (c, r, (o, ev)) in enumerate(zip(cfgs, b.results(), batch_oracle)):
+        same = identical(r.els, o)
+        bad = int((~same).sum())
+        assert bad <= max(1, math.ceil(1e-3 * c["n"])), (i, _name(c), bad, c["n"])
+        check_values(r.els, o, same)
+        n_same += int(same.sum())
+        n_all += c["n"]
+    print(f"identical trajectories: {n_same}/{n_all} = {n_same / n_all:.6f}")
+    assert n_same / n_all >= 0.999
 
 
 @pytest.mark.parametrize("use_graph,guard", [(False, None), (True, None), (True, "deferred"), (False, "service"),
@@ -120,21 +182,12 @@ def test_batch_with_thin_film_and_max_scatt_items():
     """Items with a finite thickness and maxScatt (DS and continuous) in one
     batch give the per-configuration records ('long' and 'trsm' outcomes
     included) in every guard form."""
-    import os
     from mcd import _abi, engine
-    old = os.environ.get("MCD_PICK_NODES")
-    os.environ["MCD_PICK_NODES"] = "128"
-    try:
-        trs = [engine.Transport("Si", 10000.0, 1000.0, Wc=100.0, s_tilt=0.0, tol_E=1e-2, tol_W=1e-2,
-                                thickness=3000.0, max_scatt=80),
-               engine.Transport("Au", 30000.0, 1000.0, s_tilt=0.0, mode="cont", bethe_model="JL", thickness=3000.0,
-                                max_scatt=150),
-               engine.Transport("Cu", 20000.0, 1000.0, Wc=100.0, s_tilt=70.0, tol_E=1e-2, tol_W=1e-2)]
-    finally:
-        if old is None:
-            del os.environ["MCD_PICK_NODES"]
-        else:
-            os.environ["MCD_PICK_NODES"] = old
+    trs = [engine.Transport("Si", 10000.0, 1000.0, Wc=100.0, s_tilt=0.0, tol_E=1e-2, tol_W=1e-2,
+                            thickness=3000.0, max_scatt=80, pick_nodes=128),
+           engine.Transport("Au", 30000.0, 1000.0, s_tilt=0.0, mode="cont", bethe_model="JL", thickness=3000.0,
+                            max_scatt=150),
+           engine.Transport("Cu", 20000.0, 1000.0, Wc=100.0, s_tilt=70.0, tol_E=1e-2, tol_W=1e-2, pick_nodes=128)]
     sizes = [3000, 2000, 4000]
     refs = [tr.run(n, seed=6, config_id=i) for i, (tr, n) in enumerate(zip(trs, sizes))]
     for out in (_abi.OUT_LONG, _abi.OUT_TRSM):
@@ -146,3 +199,43 @@ def test_batch_with_thin_film_and_max_scatt_items():
         for i, (r, ref) in enumerate(zip(b.results(), refs)):
             assert r.els.tobytes() == ref.els.tobytes(), (guard, i)
             assert int(r.counters[_abi.CNT_EVENTS]) == ref.events, (guard, i)
+
+
+def test_full_size_batch_properties():
+    """The full 64-configuration workload (14.67 M electrons, log-uniform
+    2^14..2^20 per configuration, seed 3) in one graph-replayed launch:
+    every item finishes each of its electrons once (counters and records
+    agree), its event count is the sum of its records' scatter counts, every
+    outcome is 'abs' or 'bks' on a semi-infinite target; each item's
+    backscatter yield is within 3 sigma of the all-FP64 per-configuration run
+    of the same streams, and the yields against independent FP32 streams
+    (seed 103) pass a chi-square test over the 64 items (p > 0.001)."""
+    from scipy import stats
+    from mcd import _abi, engine, workloads
+    cfgs = workloads.batched_configs()
+    trs = workloads.build_transports(cfgs, pick_nodes=128)
+    b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs, cfgs)], pick_nodes=128)
+    b.launch(seed=3, use_graph=True)
+    res = b.results()
+    chi2 = 0.0
+    for i, (tr, c, r) in enumerate(zip(trs, cfgs, res)):
+        n = c["n"]
+        cnt = r.counters
+        assert int(cnt[_abi.CNT_ELECTRONS]) == n, (i, _name(c))
+        assert int(cnt[_abi.CNT_EVENTS]) == int(r.els["n_scatt"].astype(np.int64).sum()), (i, _name(c))
+        out = r.els["outcome"]
+        assert np.isin(out, (_abi.OUT_ABS, _abi.OUT_BKS)).all(), (i, _name(c))
+        nb = int((out == _abi.OUT_BKS).sum())
+        assert int(cnt[_abi.CNT_BKS]) == nb, (i, _name(c))
+        y = nb / n
+        r64 = tr.run(n, seed=3, config_id=i, precision=_abi.PREC_FP64)
+        y64 = float((r64.els["outcome"] == _abi.OUT_BKS).mean())
+        sig = math.sqrt(max(y64 * (1 - y64), 1e-12) / n)
+        assert abs(y - y64) < 3 * sig, (i, _name(c), y, y64, sig)
+        ind = tr.run(n, seed=103, config_id=i)
+        yi = float((ind.els["outcome"] == _abi.OUT_BKS).mean())
+        chi2 += (y - yi) ** 2 / max(y * (1 - y) / n + yi * (1 - yi) / n, 1e-300)
+        print(f"item {i}: {_name(c)} n={n} yield {y:.5f} fp64 {y64:.5f} independent {yi:.5f}", flush=True)
+    p = stats.chi2.sf(chi2, len(cfgs))
+    print(f"chi2 = {chi2:.1f} over {len(cfgs)} items, p = {p:.3g}")
+    assert p > 1e-3

@@ -55,6 +55,111 @@ def test_multitraj_cont_signature_and_dicts():
     assert len(res) == 4 and all(r["els"]["outcome"][0] in ("abs", "bks", "trsm") for r in res)
 
 
+NAMES = {1: "abs", 2: "bks", 3: "trsm", 5: "long"}
+
+
+def oracle_columns(o, trace, material):
+    """The zipped els / scats columns the reference's listener would append for
+    oracle records and trace (thingsToSave + zipDict, fileTools.py:67-99,
+    193-242): an electron that exits and is absorbed in one step has two els
+    rows (electron.py:52-54)."""
+    from mcd.fileTools import channel_names
+    els = {k: [] for k in ("outcome", "final_E", "dx", "dy", "dz", "last_x", "last_y", "last_z")}
+    for e in o:
+        outs = [e["outcome"]] + ([e["outcome2"]] if e["outcome2"] else [])
+        for oc in outs:
+            els["outcome"].append(NAMES[int(oc)])
+            els["final_E"].append(float(e["final_E"]))
+            for j, k in enumerate(("dx", "dy", "dz")):
+                els[k].append(float(e["dir"][j]))
+            for j, k in enumerate(("last_x", "last_y", "last_z")):
+                els[k].append(float(e["pos"][j]))
+    names = channel_names(material)
+    scats = {"type": [names[int(c)] for c in trace["channel"]], "E": list(trace["E"]),
+             "E_loss": list(trace["E_loss"]), "pol_angle": list(trace["pol"]), "az_angle": list(trace["az"]),
+             "pathl": list(trace["pathl"]), "energy": list(trace["energy"]),
+             "x": list(trace["x"]), "y": list(trace["y"]), "z": list(trace["z"])}
+    return els, scats
+
+
+def compare_columns(got_els, got_scats, want_els, want_scats):
+    assert list(got_els["outcome"]) == want_els["outcome"]
+    assert np.allclose(got_els["final_E"], want_els["final_E"], rtol=1e-5)
+    for k in ("dx", "dy", "dz"):
+        assert np.abs(np.asarray(got_els[k]) - want_els[k]).max() < 1e-5, k
+    for k in ("last_x", "last_y", "last_z"):
+        assert np.allclose(got_els[k], want_els[k], rtol=1e-5, atol=1e-2), k
+    assert list(got_scats["type"]) == want_scats["type"]
+    assert np.array_equal(np.asarray(got_scats["E_loss"], np.float32), np.asarray(want_scats["E_loss"], np.float32))
+    for k, rtol in (("E", 1e-6), ("energy", 1e-6), ("az_angle", 1e-6), ("pathl", 2e-5)):
+        assert np.allclose(got_scats[k], want_scats[k], rtol=rtol), k
+    assert np.allclose(got_scats["pol_angle"], want_scats["pol_angle"], rtol=1e-5, atol=1e-6)
+    for k in ("x", "y", "z"):
+        assert np.allclose(got_scats[k], want_scats[k], rtol=1e-5, atol=1e-2), k
+
+
+ALL_SCATS = ["type", "E", "E_loss", "pol_angle", "az_angle", "pathl", "position", "energy"]
+
+
+def test_scatter_multi_el_ds_matches_oracle():
+    """The legacy DS entry point scatterMultiEl_DS (multiScatter.py:15-61):
+    inputPar['num_el'] trajectories, zipped els / scats columns in outputQ,
+    against the oracle's records and per-step trace of the same electrons."""
+    from mcd.material import Material
+    from mcd.multiScatter import genTables, scatterMultiEl_DS
+    p = si_input(num_el=48, seed=5)
+    tables = genTables(p)
+    q = scatterMultiEl_DS(p, tables, {"el_output": p["electron_output"], "scat_output": ALL_SCATS}, {})
+    tr = tables.transport(p)
+    gm, gg = tr.grids
+    o, trace, _ = O.run(tr.cfg, O.Table(gm.Es, gm.Ws, tr.cdf(0)), O.Table(gg.Es, gg.Ws, tr.cdf(1)), 5, 0, 48,
+                        trace_cap=200000)
+    want_els, want_scats = oracle_columns(o, trace, Material("Si"))
+    compare_columns(q["els"], q["scats"], want_els, want_scats)
+
+
+@pytest.mark.parametrize("bethe", ["classical", "JL", "explicit"])
+def test_scatter_multi_el_cont_matches_oracle(bethe):
+    """The legacy continuous entry point scatterMultiEl_cont
+    (multiScatter.py:118-168; its reference body fails with a NameError at
+    :131) with each Bethe law, against the oracle's continuous trajectories."""
+    from mcd.material import Material
+    from mcd.multiScatter import scatterMultiEl_cont
+    p = si_input(mode="cont", Bethe_model=bethe, seed=7)
+    things = {"el_output": p["electron_output"], "scat_output": ALL_SCATS}
+    out = scatterMultiEl_cont(p, things, {}, 40)
+    from mcd import _abi
+    cfg = _abi.make_config(Material("Si"), p["E0"], p["Emin"], p["Wc"], p["s_tilt"], mode="cont",
+                           bethe_model=bethe)
+    o, trace, _ = O.run(cfg, None, None, 7, 0, 40, trace_cap=400000)
+    want_els, want_scats = oracle_columns(o, trace, Material("Si"))
+    compare_columns(out["electrons"], out["scatterings"], want_els, want_scats)
+
+
+def test_multitraj_cont_matches_oracle():
+    """multiTraj_cont (multiScatter.py:171-229, unreachable in the reference:
+    its call at :206-207 passes 5 arguments to a 6-parameter function) returns
+    per-electron dicts equal to the oracle's continuous trajectories; two
+    calls on one tables object continue the electron numbering."""
+    from mcd.material import Material
+    from mcd.multiScatter import genTables, multiTraj_cont
+    from mcd.fileTools import zipDict
+    p = si_input(mode="cont", Bethe_model="JL", seed=9)
+    tables = genTables(p)
+    things = {"el_output": p["electron_output"], "scat_output": ALL_SCATS}
+    res = multiTraj_cont(p, 4, Material("Si"), things, tables) + multiTraj_cont(p, 4, Material("Si"), things, tables)
+    assert len(res) == 8
+    els, scats = {}, {}
+    for r in res:
+        zipDict(els, r["els"])
+        zipDict(scats, r["scats"])
+    from mcd import _abi
+    cfg = _abi.make_config(Material("Si"), p["E0"], p["Emin"], p["Wc"], p["s_tilt"], mode="cont", bethe_model="JL")
+    o, trace, _ = O.run(cfg, None, None, 9, 0, 8, trace_cap=200000)
+    want_els, want_scats = oracle_columns(o, trace, Material("Si"))
+    compare_columns(els, scats, want_els, want_scats)
+
+
 def test_doscatter_cli_writes_tables(tmp_path):
     import pandas as pd
     import doScatter

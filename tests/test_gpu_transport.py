@@ -7,7 +7,8 @@ import pytest
 
 import oracle as O
 import tables_ref as TR
-from conftest import CU20, SI10
+from conftest import CU20, SI10, check_values, identical
+from divergence import classify_divergence
 from mcd import _abi
 
 pytestmark = pytest.mark.gpu
@@ -25,22 +26,6 @@ def oracle_for(tr):
     return O.Table(gm.Es, gm.Ws, tr.cdf(0)), O.Table(gg.Es, gg.Ws, tr.cdf(1))
 
 
-def identical(g, o):
-    return (g["outcome"] == o["outcome"]) & (g["outcome2"] == o["outcome2"]) & (g["n_scatt"] == o["n_scatt"])
-
-
-def check_values(g, o, same, rel_E=1e-5, dir_abs=1e-5, pos_rel=1e-5):
-    """Energies and directions of identical trajectories within tolerance; the
-    position error is measured against the trajectory's length scale
-    (mean free path x events), since a coordinate can sit near zero."""
-    E_err = np.abs(g["final_E"][same] - o["final_E"][same]) / o["final_E"][same]
-    assert E_err.max() <= rel_E, E_err.max()
-    d_err = np.abs(g["dir"][same] - o["dir"][same]).max()
-    assert d_err <= dir_abs, d_err
-    scale = np.maximum(np.linalg.norm(o["pos"][same], axis=1), 40.0 * np.sqrt(o["n_scatt"][same] + 1.0))
-    p_err = (np.linalg.norm(g["pos"][same] - o["pos"][same], axis=1) / scale).max()
-    assert p_err <= pos_rel, p_err
-    return E_err.max(), d_err, p_err
 
 
 @pytest.fixture(scope="module")
@@ -175,6 +160,44 @@ def test_headline_config_first_1e4_electrons(cu20):
     check_values(r.els, o, same)
     r64 = tr.run(n, seed=1, precision=_abi.PREC_FP64)
     assert identical(r64.els, o).all()
+
+
+def test_headline_first_1e4_electrons_per_step(cu20):
+    """The headline workload (Cu 20 keV, 70 deg, Wc 50 eV, tol 1e-3, seed 1),
+    its first 10^4 electrons step by step against the oracle trace
+    (scattering.py:39-59, singleScatter.py:10-41): the FP64 kernel's scatter
+    rows equal the trace (channel, loss, outcome and energies bit for bit as
+    float32); the FP32 kernel's rows leave the trace only where an FP32
+    position or direction decides an exit, or where u ties a boundary, never
+    through a missed guard-band pick or a different table lookup."""
+    tr, tm, tg = cu20
+    n, chunk = 10000, 2500
+    g64, g32, otr = [], [], []
+    for first in range(0, n, chunk):
+        _, t, _ = O.run(tr.cfg, tm, tg, 1, first, chunk, trace_cap=chunk * 2500)
+        otr.append(t)
+        g64.append(tr.run(chunk, seed=1, first=first, scats=True, precision=_abi.PREC_FP64).scats)
+        g32.append(tr.run(chunk, seed=1, first=first, scats=True).scats)
+    o = np.concatenate(otr)
+    r64 = np.concatenate(g64)
+    r32 = np.concatenate(g32)
+    print(f"{len(o)} oracle rows, {len(r64)} FP64 rows, {len(r32)} FP32 rows", flush=True)
+    assert len(r64) == len(o)
+    assert np.array_equal(r64["electron"], o["electron"]) and np.array_equal(r64["step"], o["step"])
+    assert np.array_equal(r64["channel"], o["channel"])
+    assert np.array_equal(r64["E_loss"], o["E_loss"].astype(np.float32))
+    assert np.array_equal(r64["outcome"], o["outcome"].astype(np.uint8))
+    assert np.array_equal(r64["E"], o["E"].astype(np.float32))
+    assert np.array_equal(r64["energy"], o["energy"].astype(np.float32))
+    cls = classify_divergence(tr.cfg, (tm, tg), r32, o)
+    counts = {}
+    for _, _, c in cls:
+        counts[c] = counts.get(c, 0) + 1
+    print(f"FP32 electrons leaving the trace: {len(cls)} of {n}; first divergence classes: {counts}")
+    for e, st, c in cls:
+        print(f"  electron {e} step {st}: {c}")
+    assert counts.get("pick_miss", 0) == 0 and counts.get("table_miss", 0) == 0, counts
+    assert len(cls) <= 0.001 * n
 
 
 def test_headline_config_full_size_statistics(cu20):
@@ -354,6 +377,9 @@ def test_max_scatt_and_thin_film_outcomes():
         assert (o["outcome"] == out).sum() > 50
     same = identical(r.els, o)
     assert same.mean() >= 0.999
+    check_values(r.els, o, same)
+    # the 'long' record carries the compensated position, like every other outcome
+    check_values(r.els, o, same & (o["outcome"] == _abi.OUT_LONG))
     for guard in ("service", "deferred"):  # the 'long' and 'trsm' paths in the other guard forms
         assert tr.run(n, seed=6, guard=guard).els.tobytes() == r.els.tobytes(), guard
 
@@ -375,6 +401,7 @@ def test_continuous_max_scatt_and_thin_film_outcomes():
     same = identical(r.els, o)
     assert same.mean() >= 0.999, same.mean()
     check_values(r.els, o, same)
+    check_values(r.els, o, same & (o["outcome"] == _abi.OUT_LONG))
     assert int(r.counters[_abi.CNT_ELECTRONS]) == n
 
 
@@ -394,15 +421,18 @@ def test_scatter_buffer_overflow_is_reported(si10):
         tr.run(64, seed=1, scats=True, scat_capacity=100)
 
 
-def test_pipelined_chunks_on_two_streams_equal_one_launch(cu20):
+@pytest.mark.parametrize("guard", [None, "deferred", "service"])
+def test_pipelined_chunks_on_two_streams_equal_one_launch(cu20, guard):
     """Launches alternating between two streams, each with its own counters
-    block (which holds its work counter), run concurrently on the device and
-    give the records of one launch over the whole range."""
+    block (which holds its work counters) and, in the deferred form, the
+    stream's own park buffer, run concurrently on the device and give the
+    records of one launch over the whole range."""
     tr = cu20[0]
     n = (1 << 18) + 5
     whole = tr.run(n, seed=3, first=11)
     got, events = [], 0
-    for start, r in tr.run_chunks(n, 1 << 16, seed=3, first=11, streams=2):
+    print(f"run_chunks guard={guard}", flush=True)
+    for start, r in tr.run_chunks(n, 1 << 16, seed=3, first=11, streams=2, guard=guard):
         assert start == 11 + sum(len(x) for x in got)
         got.append(r.els)
         events += int(r.counters[_abi.CNT_EVENTS])

@@ -83,7 +83,8 @@ def test_python_constants_match_the_header():
             "MCD_LAUNCH_SERVICE_GUARD": _abi.LAUNCH_SERVICE_GUARD, "MCD_FLAG_FP64_PICK": _abi.FLAG_FP64_PICK,
             "MCD_FLAG_GUARD_TIMEOUT": _abi.FLAG_GUARD_TIMEOUT, "MCD_CNT_EVENTS": _abi.CNT_EVENTS,
             "MCD_CNT_FP64_PICKS": _abi.CNT_FP64_PICKS, "MCD_CNT_QUEUE": _abi.CNT_QUEUE,
-            "MCD_CNT_GUARD_TIMEOUTS": _abi.CNT_GUARD_TIMEOUTS, "MCD_LAUNCH_REFILL_SHIFT": _abi.LAUNCH_REFILL_SHIFT}
+            "MCD_CNT_GUARD_TIMEOUTS": _abi.CNT_GUARD_TIMEOUTS, "MCD_CNT_PARKED": _abi.CNT_PARKED,
+            "MCD_CNT_RESUMED": _abi.CNT_RESUMED, "MCD_LAUNCH_REFILL_SHIFT": _abi.LAUNCH_REFILL_SHIFT}
     for k, v in want.items():
         assert vals[k] == v, k
 

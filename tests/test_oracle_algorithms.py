@@ -284,3 +284,53 @@ def test_oracle_and_table_restatement_reproduce_the_committed_golden_vectors():
     for k in ref.files:
         assert np.array_equal(ref[k], now[k]), k
     assert len(ref["trace_step"]) > 1000 and set(ref["els_outcome"]) <= {1, 2}
+
+
+def test_divergence_classifier_on_planted_rows(si10):
+    """The per-step parity test's classifier, on kernel-like rows made from
+    an oracle trace with one planted difference per electron: a channel
+    changed where u is far from every boundary (a missed guard pick), a
+    channel changed where u sits on a boundary (a tie), a different loss
+    (a table miss), a different outcome (an exit flip) and a shortened
+    trajectory; unchanged electrons are not reported."""
+    import oracle as O
+    from divergence import classify_divergence
+    from mcd import _abi
+    from mcd.material import Material
+    m = Material("Si")
+    cfg = _abi.make_config(m, si10["E0"], si10["Emin"], si10["Wc"], si10["s_tilt"])
+    _, tr, _ = O.run(cfg, si10["tm"], si10["tg"], 4, 0, 8, trace_cap=100000)
+    g = np.zeros(len(tr), dtype=_abi.SCATTER_DTYPE)
+    for k in ("electron", "step", "E", "E_loss", "pol", "az", "pathl", "x", "y", "z", "energy", "dx", "dy", "dz"):
+        g[k] = tr[k]
+    g["channel"] = tr["channel"]
+    g["outcome"] = tr["outcome"]
+    o = tr.copy()
+    assert classify_divergence(cfg, (si10["tm"], si10["tg"]), g, o) == []
+
+    def row(e, st):
+        return int(np.flatnonzero((tr["electron"] == e) & (tr["step"] == st))[0])
+
+    # electron 1, step 3: a channel swap far from any boundary -> pick_miss
+    i = row(1, 3)
+    g["channel"][i] = _abi.CH_QUINN if g["channel"][i] != _abi.CH_QUINN else _abi.CH_RUTH
+    # electron 2, step 4: the oracle's u put exactly on a boundary, channel swapped -> pick_tie
+    i = row(2, 4)
+    sig, _ = O.sigma_set(cfg, float(o["E"][i]))
+    srt = sorted(range(len(sig)), key=lambda q: -sig[q])
+    o["u"][i, 0] = float(sig[srt[0]]) / sum(float(sig[q]) for q in srt)
+    g["channel"][i] = _abi.CH_QUINN if g["channel"][i] != _abi.CH_QUINN else _abi.CH_RUTH
+    # the first other electron with a Moller or Gryzinski step: another loss there -> table_miss
+    lossy = (tr["channel"] == _abi.CH_MOLLER) | (tr["channel"] >= _abi.CH_GRYZ0)
+    e3 = next(e for e in (3, 4, 7, 0) if np.any(lossy & (tr["electron"] == e)))
+    i = row(e3, int(tr["step"][lossy & (tr["electron"] == e3)][0]))
+    g["E_loss"][i] = g["E_loss"][i] * 2 + 1
+    # electron 5: the last step without its outcome -> exit
+    i = row(5, int(tr["step"][tr["electron"] == 5].max()))
+    g["outcome"][i] = 0
+    # electron 6: its last row missing -> length
+    keep = np.ones(len(g), bool)
+    keep[row(6, int(tr["step"][tr["electron"] == 6].max()))] = False
+    got = classify_divergence(cfg, (si10["tm"], si10["tg"]), g[keep], o)
+    classes = {e: c for e, _, c in got}
+    assert classes == {1: "pick_miss", 2: "pick_tie", e3: "table_miss", 5: "exit", 6: "length"}, got

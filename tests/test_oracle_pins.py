@@ -1,0 +1,226 @@
+"""Independent pins of the oracle's Joy-Luo and explicit Bethe stopping powers
+and of its Quinn plasmon cross section.
+
+The reference's JL and explicit laws (MC/tools/stoppingPowers.py:145-173,
+177-215) are never reached by its own code and none of its tests evaluates
+them, and its Quinn cross section (MC/tools/crossSections.py:247-284) has no
+numeric test either.  Each law is therefore tied here to something that is not
+its own formula:
+
+* JL: with k = 0 it is the classical law, which the textbook engineering form
+  pins (test_oracle_rng_physics); for E >> kJ its excess over the classical law
+  is the first-order term 2 pi e^4 n Z / E * kJ / E; and it equals Joy and
+  Luo's engineering form 78500 rho Z / (A E) ln(1.166 (E + kJ) / J) keV/cm
+  (Scanning 11, 176 (1989)).
+* explicit: with one valence term (n = Z, E_val) and no inner shells it is the
+  classical law with J = 1.166 E_val; splitting a shell in two halves at the
+  same binding energy leaves it unchanged; each shell adds
+  2 pi e^4 n n_i / E ln(E / E_i).
+* Quinn: the inverse mean free path E_pl / (2 a0 E) ln(q_c / q_min) rebuilt
+  from electron wave vectors (q_min = k - k' for a loss E_pl, q_c =
+  k_F (sqrt(1 + E_pl / E_F) - 1)) with scipy's CODATA constants; its
+  E >> E_F asymptote with q_min -> E_pl / (2 sqrt(E E_F)) k_F; zero at and
+  below the plasmon threshold.
+
+Every pin is written as a predicate and also applied to deliberately wrong
+variants of the law (a dropped factor, a swapped argument, a wrong sign) to
+show that the pin rejects them.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+from scipy.constants import e as E_CHARGE, hbar, m_e
+
+import oracle as O
+from mcd import physics
+from mcd.material import Material, bethe_J
+
+PI_EFOUR = physics.PI_EFOUR
+BOHR_R = 5.29177210903e-11  # m, parameters.py:8 (scipy CODATA 2018)
+
+
+def S_cl(Z, E, n, J):
+    return O.lib().oracle_bethe_cl(int(Z), float(E), float(n), float(J))
+
+
+def S_jl(Z, E, n, k, J):
+    return O.lib().oracle_bethe_jl(int(Z), float(E), float(n), float(k), float(J))
+
+
+def S_expl(E, n, shells, Zval, Eval):
+    Zi = (ctypes.c_double * max(1, len(shells)))(*[s[0] for s in shells])
+    Ei = (ctypes.c_double * max(1, len(shells)))(*[s[1] for s in shells])
+    return O.lib().oracle_bethe_explicit(float(E), float(n), Zi, Ei, len(shells), float(Zval), float(Eval))
+
+
+def sig_quinn(E, Epl, Ef, n):
+    return O.lib().oracle_sigma_quinn(float(E), float(Epl), float(Ef), float(n))
+
+
+# ---------------------------------------------------------------------------
+# Joy-Luo Bethe law
+
+def pin_jl(jl, cl=S_cl):
+    """True when `jl(Z, E, n, k, J)` passes every JL pin."""
+    for sp in ("Al", "Cu", "Au"):
+        m = Material(sp)
+        Z, n, k, J = m.Z, m.atnd, m.params["bethe_k"], bethe_J(m.Z)
+        for E in (1500.0, 5000.0, 20000.0):
+            if jl(Z, E, n, 0.0, J) != cl(Z, E, n, J):  # k = 0: the classical law, bit for bit
+                return False
+        # E >> kJ: S_JL - S_cl = 2 pi e^4 n Z / E ln(1 + kJ/E) ~ 2 pi e^4 n Z / E * kJ / E
+        E = 1e8
+        first = 2 * PI_EFOUR * n * Z / E * (k * J / E) * 1e-14
+        if not math.isclose(jl(Z, E, n, k, J) - cl(Z, E, n, J), first, rel_tol=1e-4):
+            return False
+        # Joy & Luo's engineering form in keV/cm, E and J in keV (1 keV/cm = 1e-5 eV/A)
+        p = m.params
+        for E in (1500.0, 5000.0, 20000.0):
+            Ek, Jk = E / 1000.0, J / 1000.0
+            S_kev_cm = 78500.0 * p["density"] * Z / (p["atwt"] * Ek) * math.log(1.166 * (Ek + k * Jk) / Jk)
+            if not math.isclose(jl(Z, E, n, k, J), S_kev_cm * 1e-5, rel_tol=2e-3):
+                return False
+    return True
+
+
+def test_joy_luo_law_pins():
+    assert pin_jl(S_jl)
+
+
+@pytest.mark.parametrize("name,mutant", [
+    ("k*J -> k", lambda Z, E, n, k, J: 2 * PI_EFOUR * n * (Z / E) * math.log(1.166 * (E + k) / J) * 1e-14),
+    ("E + kJ -> E - kJ", lambda Z, E, n, k, J: 2 * PI_EFOUR * n * (Z / E) * math.log(1.166 * (E - k * J) / J) * 1e-14),
+    ("dropped 2", lambda Z, E, n, k, J: PI_EFOUR * n * (Z / E) * math.log(1.166 * (E + k * J) / J) * 1e-14),
+    ("1.166 on kJ only", lambda Z, E, n, k, J: 2 * PI_EFOUR * n * (Z / E) * math.log((E + 1.166 * k * J) / J) * 1e-14),
+    ("E and J swapped in the log", lambda Z, E, n, k, J: 2 * PI_EFOUR * n * (Z / E) * math.log(1.166 * (J + k * E) / E)
+     * 1e-14),
+])
+def test_joy_luo_pins_reject_planted_mistakes(name, mutant):
+    assert not pin_jl(mutant), name
+
+
+# ---------------------------------------------------------------------------
+# explicit Bethe law (shell sum)
+
+def pin_explicit(expl, cl=S_cl):
+    """True when `expl(E, n, shells, Zval, Eval)` passes every explicit-law pin
+    (shells: list of (occupancy, binding energy))."""
+    for sp in ("Si", "Cu", "Au"):
+        m = Material(sp)
+        Z, n = m.Z, m.atnd
+        J = bethe_J(Z)
+        for E in (1000.0, 5000.0, 30000.0):
+            # one valence term holding all Z electrons, no shells: the classical law at J = 1.166 E_val
+            if not math.isclose(expl(E, n, [], Z, J / 1.166), cl(Z, E, n, J), rel_tol=1e-12):
+                return False
+            shells = [(s[2], s[1]) for s in m.shells]
+            full = expl(E, n, shells, m.params["n_val"], m.params["E_val"])
+            # a shell split into two halves at the same binding energy changes nothing
+            split = [(shells[0][0] / 2, shells[0][1]), (shells[0][0] / 2, shells[0][1])] + shells[1:]
+            if not math.isclose(expl(E, n, split, m.params["n_val"], m.params["E_val"]), full, rel_tol=1e-12):
+                return False
+            # each shell adds 2 pi e^4 n n_i / E ln(E / E_i)
+            for i, (ni, Ei) in enumerate(shells):
+                less = expl(E, n, shells[:i] + shells[i + 1:], m.params["n_val"], m.params["E_val"])
+                add = 2 * PI_EFOUR * n / E * 1e-14 * ni * math.log(E / Ei)
+                if not math.isclose(full - less, add, rel_tol=1e-9, abs_tol=1e-12 * abs(full)):
+                    return False
+    return True
+
+
+def _expl(E, n, shells, Zval, Eval, *, f=lambda E, Ei: math.log(E / Ei), pref=2.0, lead=True):
+    s = Zval * f(E, Eval) if lead else 0.0
+    for ni, Ei in shells:
+        s += ni * f(E, Ei)
+    return pref * PI_EFOUR * n / E * 1e-14 * s
+
+
+def test_explicit_law_pins():
+    assert pin_explicit(S_expl)
+
+
+@pytest.mark.parametrize("name,mutant", [
+    ("valence term dropped", lambda E, n, sh, Zv, Ev: _expl(E, n, sh, Zv, Ev, lead=False)),
+    ("ratio inverted", lambda E, n, sh, Zv, Ev: _expl(E, n, sh, Zv, Ev, f=lambda E, Ei: math.log(Ei / E))),
+    ("dropped 2", lambda E, n, sh, Zv, Ev: _expl(E, n, sh, Zv, Ev, pref=1.0)),
+    ("1.166 inside the shell logs", lambda E, n, sh, Zv, Ev: _expl(E, n, sh, Zv, Ev,
+                                                                   f=lambda E, Ei: math.log(1.166 * E / Ei))),
+    ("occupancy and energy swapped", lambda E, n, sh, Zv, Ev: _expl(E, n, [(b, a) for a, b in sh], Zv, Ev)),
+])
+def test_explicit_pins_reject_planted_mistakes(name, mutant):
+    assert not pin_explicit(mutant), name
+
+
+# ---------------------------------------------------------------------------
+# Quinn plasmon cross section
+
+def quinn_from_wave_vectors(E, Epl, Ef, n):
+    """sigma = 1 / (lambda n) with 1/lambda = E_pl / (2 a0 E) ln(q_c / q_min),
+    the wave vectors built in SI units from the energies [eV]; returns cm^2."""
+    k = math.sqrt(2 * m_e * E * E_CHARGE) / hbar
+    kp = math.sqrt(2 * m_e * (E - Epl) * E_CHARGE) / hbar
+    kF = math.sqrt(2 * m_e * Ef * E_CHARGE) / hbar
+    q_min = k - kp
+    q_c = kF * (math.sqrt(1 + Epl / Ef) - 1)
+    inv_lambda = Epl / (2 * BOHR_R * E) * math.log(q_c / q_min)  # 1/m
+    return inv_lambda / n * 1e4
+
+
+def pin_quinn(sig):
+    """True when `sig(E, Epl, Ef, n)` passes every Quinn pin."""
+    for sp in ("Al", "Si", "Cu", "Au"):
+        m = Material(sp)
+        Epl, Ef, n = m.plasmon_e, m.fermi_e, m.atnd
+        for E in (5 * Epl, 1000.0, 20000.0, 1e6):
+            if not math.isclose(sig(E, Epl, Ef, n), quinn_from_wave_vectors(E, Epl, Ef, n), rel_tol=1e-9):
+                return False
+        # E >> E_F: q_min / k_F -> E_pl / (2 sqrt(E E_F)), so the log argument is
+        # (sqrt(1 + E_pl/E_F) - 1) 2 sqrt(E E_F) / E_pl up to O(E_pl / E)
+        E = 1e6
+        asym = Epl / (2 * BOHR_R * E) * math.log((math.sqrt(1 + Epl / Ef) - 1) * 2 * math.sqrt(E * Ef) / Epl) / n * 1e4
+        if not math.isclose(sig(E, Epl, Ef, n), asym, rel_tol=Epl / E):
+            return False
+        # no plasmon below threshold: zero at and below E_pl, positive well above it
+        if sig(Epl, Epl, Ef, n) != 0.0 or sig(0.5 * Epl, Epl, Ef, n) != 0.0:
+            return False
+        if not sig(10 * Epl, Epl, Ef, n) > 0.0:
+            return False
+    return True
+
+
+def _quinn(E, Epl, Ef, n, *, two=2.0, swap=False, cm2=1e4):
+    if E <= Epl:
+        return 0.0
+    a, b = (Ef, Epl) if swap else (Epl, Ef)
+    x, p = E / b, a / b
+    return a * math.log((math.sqrt(1 + p) - 1) / (math.sqrt(x) - math.sqrt(x - p))) / (two * BOHR_R * E) / n * cm2
+
+
+def test_quinn_pins():
+    assert pin_quinn(sig_quinn)
+
+
+@pytest.mark.parametrize("name,mutant", [
+    ("E_pl and E_F swapped", lambda E, Epl, Ef, n: _quinn(E, Epl, Ef, n, swap=True)),
+    ("dropped 2", lambda E, Epl, Ef, n: _quinn(E, Epl, Ef, n, two=1.0)),
+    ("m^2 -> cm^2 factor missing", lambda E, Epl, Ef, n: _quinn(E, Epl, Ef, n, cm2=1.0)),
+    ("no threshold", lambda E, Epl, Ef, n: _quinn(E, Epl * 0.999999, Ef, n) if E > 0.5 * Epl else 0.0),
+])
+def test_quinn_pins_reject_planted_mistakes(name, mutant):
+    assert not pin_quinn(mutant), name
+
+
+def test_quinn_mean_free_path_scale():
+    """Plasmon mean free paths of the order of 10^2 A at 10 keV, growing
+    about linearly with E (the E_pl / E prefactor with a slowly varying
+    log), as measured for Al and Si (Egerton, EELS in the Electron
+    Microscope, 3rd ed., Sec. 3.3: ~100-200 nm at 100 keV)."""
+    for sp in ("Al", "Si"):
+        m = Material(sp)
+        lam = lambda E: 1.0 / (m.atnd * 1e-6 * sig_quinn(E, m.plasmon_e, m.fermi_e, m.atnd)) * 1e8  # noqa: E731
+        assert 50.0 < lam(10000.0) < 400.0
+        assert 1000.0 < lam(100000.0) < 3000.0
+        r = lam(20000.0) / lam(10000.0)
+        assert 1.7 < r < 2.0
