@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--electrons", type=int, default=1 << 22)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of each CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="target wall time of each CPU-baseline sample (samples hold at least 2^14 electrons)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=0)
@@ -58,15 +59,30 @@ def parse():
     ap.add_argument("--config-sweep", action="store_true",
                     help="instead of the headline line, print one JSON line per parity configuration "
                          "(FP64 Si 10 keV, DS Cu 20 keV, continuous Au 30 keV x 3 Bethe laws x 2 tilts)")
+    ap.add_argument("--valu-ceiling", action="store_true",
+                    help="instead of the headline line, time the issue-rate calibration kernels (v_fma_f32, "
+                         "v_mad_u64_u32, v_bitop3_b32 at full occupancy) and print one JSON line")
+    ap.add_argument("--precision", action="store_true",
+                    help="instead of the headline line, run the precision study (FP32 vs the all-FP64 GPU run of "
+                         "the same streams; DS Cu 20 keV and continuous Au 30 keV x 3 Bethe laws x 2 tilts) and "
+                         "print one JSON line per workload")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one device-to-device copy of this size (the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE readings)")
     return ap.parse_args()
 
 
-def cpu_threads():
-    n = os.environ.get("OMP_NUM_THREADS")
-    return max(1, int(n)) if n and n.isdigit() else max(1, min(16, os.cpu_count() or 1))
+def cpu_cores():
+    """(threads used, host summary): every core this process may run on
+    (sched_getaffinity), or fewer when OMP_NUM_THREADS states the process's
+    CPU share (a shared GPU host gives each GPU's jobs such a share); the
+    summary reports os.cpu_count() and the affinity beside it."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    n = aff
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n), {"os_cpu_count": os.cpu_count(), "affinity": aff, "omp_num_threads": env}
 
 
 def cpu_baselines(tr, seconds, seed):
@@ -79,28 +95,32 @@ def cpu_baselines(tr, seconds, seed):
     gm, gg = tr.grids
     tm = O.Table(gm.Es, gm.Ws, tr.cdf(0))
     tg = O.Table(gg.Es, gg.Ws, tr.cdf(1))
-    threads = cpu_threads()
-    # C oracle: grow the sample until it takes about `seconds`
-    n, ev, dt = 1024, 0, 0.0
+    threads, host = cpu_cores()
+    # C oracle: at least 2^14 electrons, grown until the sample takes about `seconds`
+    n, ev, dt = 1 << 14, 0, 0.0
     while True:
         t = time.perf_counter()
         _, _, ev = O.run(tr.cfg, tm, tg, seed, 0, n, nthreads=threads)
         dt = time.perf_counter() - t
-        if dt >= seconds or n >= (1 << 22):
+        if dt >= 0.8 * seconds or n >= (1 << 24):
             break
-        n = int(n * min(8.0, max(2.0, 1.2 * seconds / max(dt, 1e-3))))
-    c_base = {"value": ev / dt, "unit": "events/s", "cores": threads, "kind": "port",
+        n = int(n * min(8.0, max(1.25, 1.05 * seconds / max(dt, 1e-3))))
+    c_base = {"value": ev / dt, "unit": "events/s", "cores": threads, "kind": "port", "host": host,
               "sample": f"FP64 C oracle (OpenMP), electrons 0..{n - 1} of the same workload, {ev} events in {dt:.1f} s"}
-    # Python reference-path restatement under multiprocessing (4 trajectories per job)
+    # Python reference-path restatement under multiprocessing (4 trajectories per
+    # job): at least 2^14 electrons, sized from a short serial probe to take about `seconds`
     P = B1.params_from(tr.cfg, tm, tg, seed=seed)
-    n1 = max(4 * threads, int(seconds * threads * 10000 / 480))
+    t = time.perf_counter()
+    B1.run_serial(P, 0, 16)
+    per_el = (time.perf_counter() - t) / 16
+    n1 = max(1 << 14, int(seconds * threads / max(per_el, 1e-6)))
     ev1, dt1, w = B1.run_pool(P, n1, workers=threads)
-    b1 = {"value": ev1 / dt1, "unit": "events/s", "cores": w, "kind": "port",
+    b1 = {"value": ev1 / dt1, "unit": "events/s", "cores": w, "kind": "port", "host": host,
           "sample": f"Python/NumPy reference-path restatement, Pool({w}) x 4 trajectories per job, "
                     f"electrons 0..{n1 - 1}, {ev1} events in {dt1:.1f} s"}
     # the same restatement on one worker (the reference's num_workers = 1) and
     # as the reference ships it (every job carries the tables, one per shell)
-    ns = max(8, int(seconds * 8))
+    ns = max(16, int(0.5 * seconds / max(per_el, 1e-6)))
     t = time.perf_counter()
     evs, _ = B1.run_serial(P, 0, ns)
     dts = time.perf_counter() - t
@@ -276,6 +296,56 @@ def config_sweep(steps):
                           "events_per_s": events / (ms * 1e-3)}), flush=True)
 
 
+def valu_ceiling(reps=5):
+    """Wavefront-instructions per second the SIMDs issue for the VALU
+    instructions that dominate the transport's arithmetic and its Philox
+    rounds, alone and in pairs, each in a kernel of known instruction count at
+    8 wavefronts per SIMD (mc-discrete_amd/csrc/mcd_calib.hip).  Timed with HIP
+    events on the launch stream; the best of `reps` launches.  A mix issuing
+    more than one instruction per CU per clock dual-issues (gfx950 SIMDs can
+    issue two VALU instructions in one quad-cycle)."""
+    import ctypes as C
+
+    import torch
+
+    L = C.CDLL(os.path.join(ROOT, "mc-discrete_amd", "mcd", "libmcd_calib.so"))
+    L.mcd_calib_launch.restype = C.c_int
+    L.mcd_calib_launch.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.mcd_calib_insts_per_wave.restype = C.c_longlong
+    L.mcd_calib_insts_per_wave.argtypes = [C.c_int, C.c_int]
+    dev = torch.device("cuda", 0)
+    n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
+    blocks = n_cu * 8  # 8 workgroups of 4 wavefronts per CU: 8 wavefronts per SIMD
+    waves = blocks * 4
+    out = torch.empty(blocks * 256 * 4, dtype=torch.uint8, device=dev)
+    s = torch.cuda.current_stream(dev)
+    res = {"workload": f"{blocks} workgroups x 256 threads ({waves} wavefronts, 8 per SIMD on {n_cu} CUs)",
+           "cus": n_cu}
+    kinds = ((0, "v_fma_f32"), (1, "v_mad_u64_u32"), (2, "v_bitop3_b32"), (3, "v_fma_f32+v_exp_f32"),
+             (4, "v_fma_f32+v_mad_u64_u32"), (5, "v_fma_f32+v_bitop3_b32"), (6, "v_fma_f32+v_alignbit_b32"),
+             (7, "v_rcp_f32"))
+    for kind, name in kinds:
+        iters = 512 if kind in (1, 4) else 2048
+        per_wave = L.mcd_calib_insts_per_wave(kind, iters)
+        best = float("inf")
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            rc = L.mcd_calib_launch(kind, blocks, iters, out.data_ptr(), s.cuda_stream)
+            e1.record(s)
+            torch.cuda.synchronize(dev)
+            if rc != 0:
+                raise RuntimeError(f"calibration kernel {name} failed to launch ({rc})")
+            best = min(best, e0.elapsed_time(e1))
+        rate = per_wave * waves / (best * 1e-3)
+        res[name] = {"insts_per_wave": per_wave, "ms": best, "wave_insts_per_s": rate,
+                     "per_cu_per_clock_at_2_4GHz": rate / (n_cu * 2.4e9)}
+    fma = res["v_fma_f32"]["wave_insts_per_s"]
+    res["fp32_fma_tflops"] = fma * 64 * 2 / 1e12
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -284,6 +354,14 @@ def main():
         sys.exit(2)
     if a.config_sweep:
         config_sweep(a.steps)
+        return
+    if a.valu_ceiling:
+        valu_ceiling()
+        return
+    if a.precision:
+        from mcd import precision
+        for name, c in precision.run_study(a.electrons):
+            print(json.dumps({"workload": name, **c}), flush=True)
         return
     import torch
 

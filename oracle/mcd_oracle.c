@@ -265,6 +265,7 @@ typedef struct oracle_els {
   uint8_t outcome2;
   uint8_t flags;
   uint8_t pad;
+  double path;       /* total free path of the trajectory (sum of l over its steps), a length scale */
 } oracle_els;
 
 typedef struct oracle_step {
@@ -402,6 +403,7 @@ static void or_trajectory(const mcd_config* c, const oracle_table* tm, const ora
       hp = OR_PI * u[2];
     }
     if (c2 != c2) s.flags |= MCD_FLAG_NAN_DIR;
+    rec->path += l;
     oracle_rotate(c2, hp, s.dir, s.y);
     if (trace && *trace_n < trace_cap) {
       oracle_step* r = &trace[*trace_n];

@@ -29,9 +29,9 @@ class OracleTable(C.Structure):
 
 ORACLE_ELS_DTYPE = np.dtype([
     ("final_E", "<f8"), ("dir", "<f8", (3,)), ("pos", "<f8", (3,)), ("n_scatt", "<u4"),
-    ("outcome", "u1"), ("outcome2", "u1"), ("flags", "u1"), ("pad", "u1"),
+    ("outcome", "u1"), ("outcome2", "u1"), ("flags", "u1"), ("pad", "u1"), ("path", "<f8"),
 ])
-assert ORACLE_ELS_DTYPE.itemsize == 64
+assert ORACLE_ELS_DTYPE.itemsize == 72
 
 ORACLE_STEP_DTYPE = np.dtype([
     ("electron", "<u4"), ("step", "<u4"), ("channel", "<i4"), ("outcome", "<i4"),

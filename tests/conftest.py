@@ -69,17 +69,35 @@ def identical(g, o):
     return (g["outcome"] == o["outcome"]) & (g["outcome2"] == o["outcome2"]) & (g["n_scatt"] == o["n_scatt"])
 
 
-def check_values(g, o, same, rel_E=1e-5, dir_abs=1e-5, pos_rel=1e-5):
+def mfp_at(cfg, E):
+    """Mean free path [A] of a configuration at energy E from the oracle's
+    sigma set (DS: every channel; continuous: Rutherford only)."""
+    import oracle as O
+    if cfg.mode == 0:
+        sig, _ = O.sigma_set(cfg, float(E))
+        tot = float(np.sum(sig))
+    else:
+        tot = O.lib().oracle_sigma_ruth(float(E), int(cfg.mat.Z))
+    return 1.0 / (cfg.mat.atnd * tot) * 1e14
+
+
+def check_values(g, o, same, rel_E=1e-5, dir_abs=1e-5, pos_rel=1e-5, length=40.0):
     """Energies and directions of identical trajectories within tolerance; the
-    position error is measured against the trajectory's length scale
-    (mean free path x events), since a coordinate can sit near zero."""
+    position error is measured against the trajectory's length scale, since a
+    coordinate can sit near zero: its total path length when the records
+    carry it (oracle records do), else `length` (a mean free path) x
+    sqrt(events)."""
     if not np.any(same):
         return 0.0, 0.0, 0.0
     E_err = np.abs(g["final_E"][same] - o["final_E"][same]) / o["final_E"][same]
     assert E_err.max() <= rel_E, E_err.max()
     d_err = np.abs(g["dir"][same] - o["dir"][same]).max()
     assert d_err <= dir_abs, d_err
-    scale = np.maximum(np.linalg.norm(o["pos"][same], axis=1), 40.0 * np.sqrt(o["n_scatt"][same] + 1.0))
+    if "path" in o.dtype.names:
+        ref = o["path"][same]
+    else:
+        ref = length * np.sqrt(o["n_scatt"][same] + 1.0)
+    scale = np.maximum(np.linalg.norm(o["pos"][same], axis=1), ref)
     p_err = (np.linalg.norm(g["pos"][same] - o["pos"][same], axis=1) / scale).max()
     assert p_err <= pos_rel, p_err
     return E_err.max(), d_err, p_err

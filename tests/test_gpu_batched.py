@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from conftest import check_values, identical
+from conftest import check_values, identical, mfp_at
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +63,7 @@ def test_every_batch_configuration_fp64_kernel_equals_oracle(small_batch, batch_
         same = identical(r64.els, o)
         assert same.all(), (i, _name(c), np.flatnonzero(~same)[:5])
         assert r64.events == ev, (i, _name(c))
-        check_values(r64.els, o, same, rel_E=1e-7, dir_abs=1e-7, pos_rel=1e-7)
+        check_values(r64.els, o, same, rel_E=1e-7, dir_abs=1e-7, pos_rel=1e-7, length=mfp_at(tr.cfg, c["E0"]))
 
 
 def test_every_batch_configuration_fp32_batched_launch_matches_oracle(small_batch, batch_oracle):
@@ -76,11 +76,13 @@ def test_every_batch_configuration_fp32_batched_launch_matches_oracle(small_batc
     b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs, cfgs)], pick_nodes=128)
     b.launch(seed=3, use_graph=True)
     n_same = n_all = 0
-    for i, (c, r, (o, ev)) in enumerate(zip(cfgs, b.results(), batch_oracle)):
+    for i, (tr, c, r, (o, ev)) in enumerate(zip(trs, cfgs, b.results(), batch_oracle)):
         same = identical(r.els, o)
         bad = int((~same).sum())
         assert bad <= max(1, math.ceil(1e-3 * c["n"])), (i, _name(c), bad, c["n"])
-        check_values(r.els, o, same)
+        errs = check_values(r.els, o, same, length=mfp_at(tr.cfg, c["E0"]))
+        print(f"item {i}: {_name(c)} n={c['n']} differing {bad}; max errors E {errs[0]:.2e} dir {errs[1]:.2e} "
+              f"pos {errs[2]:.2e}", flush=True)
         n_same += int(same.sum())
         n_all += c["n"]
     print(f"identical trajectories: {n_same}/{n_all} = {n_same / n_all:.6f}")
