@@ -271,6 +271,24 @@ int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, u
                 const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch,
                 void* stream);
 
+/* mcd_forward that allocates nothing: no reference counterpart; for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, records and
+ * counters as mcd_forward.  A DS FP32 launch runs the deferred guard unless a
+ * guard flag asks for another form, and parks its electrons in the library's
+ * static device scratch (MCD_NOALLOC_MAX_ELECTRONS park records, ~72 MiB held
+ * by each GPU the library runs on until the process ends); every other launch
+ * form uses no scratch.  n above MCD_NOALLOC_MAX_ELECTRONS is rejected with
+ * MCD_ERR_INVALID (split the range: records are keyed by electron id).  All
+ * calls on one GPU share the scratch, so calls on one GPU must not overlap in
+ * time (one stream, or synchronise between them); the work counters are the
+ * caller's counters block and are cleared by the call with a fill kernel. */
+#define MCD_NOALLOC_MAX_ELECTRONS (1 << 20)
+int mcd_forward_noalloc(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                        uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
+                        const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch,
+                        void* stream);
+
 /* Batched launches (no reference counterpart: the reference runs one
  * configuration per process, doScatter.py:155-252).  Item i uses Philox
  * counter word config = i and electrons [0, n). */

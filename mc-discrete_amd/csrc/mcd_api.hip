@@ -24,11 +24,20 @@ struct ParkBuf {
   uint64_t cap;  // electrons
 };
 
+// Device scratch of mcd_forward_noalloc: the deferred guard's park buffer for
+// up to MCD_NOALLOC_MAX_ELECTRONS electrons, in a file-scope device array that
+// each GPU the library runs on holds until the process ends.  Host code never
+// uses the array's name as an address: the context keeps the device address
+// from hipGetSymbolAddress (ctx->scratch) and passes that to kernels.
+constexpr size_t kScratchBytes = (size_t)MCD_NOALLOC_MAX_ELECTRONS * sizeof(ParkRec);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct mcd_ctx {
   int device;
   int n_cu;
   unsigned long long* d_queue;  // [0] backward work counter
   std::vector<ParkBuf> parks;   // deferred-guard park buffers, one per stream
+  ParkRec* scratch;             // this device's address of `scratch` (mcd_forward_noalloc)
   char err[512];
 };
 
@@ -105,6 +114,15 @@ extern "C" int mcd_ctx_create(int device, mcd_ctx** out) {
     delete c;
     return set_err(nullptr, MCD_ERR_NOMEM, "mcd_ctx_create: hipMalloc of the work counters failed");
   }
+  // the device address of the static scratch array on this device (current above)
+  void* sp = nullptr;
+  const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  if (se != hipSuccess || !sp) {
+    (void)hipFree(c->d_queue);
+    delete c;
+    return set_err(nullptr, MCD_ERR_DEVICE, "mcd_ctx_create: hipGetSymbolAddress(scratch): %s", hipGetErrorString(se));
+  }
+  c->scratch = static_cast<ParkRec*>(sp);
   *out = c;
   return MCD_OK;
 }
@@ -515,9 +533,14 @@ static bool same_config(const mcd_config& a, const mcd_config& b) {
          a.Emin == b.Emin && a.Wc == b.Wc && a.mat.n_shells == b.mat.n_shells;
 }
 
-extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
-                           uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
-                           const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch, void* stream) {
+// static_park: nullptr for mcd_forward (the deferred guard parks in the
+// stream's buffer, allocated on demand); the context's scratch address for
+// mcd_forward_noalloc (DS FP32 launches then default to the deferred guard
+// and park there; nothing is allocated).
+static int forward_impl(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                        uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
+                        const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch, void* stream,
+                        ParkRec* static_park) {
   if (!ctx) return set_err(nullptr, MCD_ERR_INVALID, "mcd_forward: ctx is NULL");
   int rc = check_config(ctx, cfg);
   if (rc) return rc;
@@ -562,13 +585,13 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   // the FP64 guard pick runs inline unless another placement is asked for
   // (DS, FP32 only; continuous mode has no guard)
   const bool ds32 = cfg->mode == MCD_MODE_DS && precision == MCD_PREC_FP32;
-  const bool deferred = ds32 && (flags & MCD_LAUNCH_DEFER_GUARD);
+  const bool deferred = ds32 && ((flags & MCD_LAUNCH_DEFER_GUARD) || (static_park && !guard_flags));
   // with no placement and no shape asked for, DS launches of at least
   // kServedDefaultMin electrons take the served guard (on MI355X the fastest
   // form from 2^21 electrons up; below it the inline form is)
   const bool shape_given = launch && (launch->block > 0 || launch->waves_per_simd > 0);
   const bool service = ds32 && ((flags & MCD_LAUNCH_SERVICE_GUARD) ||
-                                (!guard_flags && !shape_given && n >= kServedDefaultMin));
+                                (!guard_flags && !shape_given && !static_park && n >= kServedDefaultMin));
   int block = launch && launch->block > 0 ? launch->block : (service ? 1024 : 256);
   if (service) {
     if (block % 64 || block < 128 || block > 1024)
@@ -590,26 +613,30 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
     HIP_TRY(ctx, launch_fill_u64(q, 0ull, 1, s));
     HIP_TRY(ctx, launch_transport_svc(c, R, minw, (int)grid, block, s));
   } else if (deferred) {
-    // the stream's park buffer (launches on other streams use their own)
-    ParkBuf* pb = nullptr;
-    for (ParkBuf& b : ctx->parks)
-      if (b.stream == s) pb = &b;
-    if (!pb) {
-      ctx->parks.push_back(ParkBuf{s, nullptr, 0});
-      pb = &ctx->parks.back();
-    }
-    if (pb->cap < n) {
-      // earlier launches on this stream may still read the buffer
-      HIP_TRY(ctx, hipStreamSynchronize(s));
-      if (pb->p) { (void)hipFree(pb->p); pb->p = nullptr; pb->cap = 0; }
-      if (hipMalloc(&pb->p, n * sizeof(ParkRec)) != hipSuccess) {
-        pb->p = nullptr;
-        return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc of the park buffer (%llu electrons) failed",
-                       (unsigned long long)n);
+    if (static_park) {
+      R.park = static_park;  // capacity MCD_NOALLOC_MAX_ELECTRONS >= n, checked by the caller
+    } else {
+      // the stream's park buffer (launches on other streams use their own)
+      ParkBuf* pb = nullptr;
+      for (ParkBuf& b : ctx->parks)
+        if (b.stream == s) pb = &b;
+      if (!pb) {
+        ctx->parks.push_back(ParkBuf{s, nullptr, 0});
+        pb = &ctx->parks.back();
       }
-      pb->cap = n;
+      if (pb->cap < n) {
+        // earlier launches on this stream may still read the buffer
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (pb->p) { (void)hipFree(pb->p); pb->p = nullptr; pb->cap = 0; }
+        if (hipMalloc(&pb->p, n * sizeof(ParkRec)) != hipSuccess) {
+          pb->p = nullptr;
+          return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc of the park buffer (%llu electrons) failed",
+                         (unsigned long long)n);
+        }
+        pb->cap = n;
+      }
+      R.park = pb->p;
     }
-    R.park = pb->p;
     // parked count and resume counter: slots of the caller's counters block
     R.park_count = (unsigned long long*)counters + MCD_CNT_PARKED;
     R.resume_queue = (unsigned long long*)counters + MCD_CNT_RESUMED;
@@ -631,6 +658,25 @@ extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables
   }
   HIP_TRY(ctx, launch_count_outcomes(els, n, (unsigned long long*)counters, s));
   return MCD_OK;
+}
+
+extern "C" int mcd_forward(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                           uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
+                           const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch, void* stream) {
+  return forward_impl(ctx, cfg, tables, seed, config_id, first, n, els, scats, counters, launch, stream, nullptr);
+}
+
+extern "C" int mcd_forward_noalloc(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* tables, uint64_t seed,
+                                   uint32_t config_id, uint64_t first, uint64_t n, mcd_electron_rec* els,
+                                   const mcd_scatter_buf* scats, uint64_t* counters, const mcd_launch* launch,
+                                   void* stream) {
+  if (!ctx) return set_err(nullptr, MCD_ERR_INVALID, "mcd_forward_noalloc: ctx is NULL");
+  if (n > (uint64_t)MCD_NOALLOC_MAX_ELECTRONS)
+    return set_err(ctx, MCD_ERR_INVALID,
+                   "mcd_forward_noalloc: %llu electrons exceed the static scratch's %d; split the range "
+                   "(records are keyed by electron id)", (unsigned long long)n, MCD_NOALLOC_MAX_ELECTRONS);
+  return forward_impl(ctx, cfg, tables, seed, config_id, first, n, els, scats, counters, launch, stream,
+                      ctx->scratch);
 }
 
 // ---------------------------------------------------------------------------

@@ -69,6 +69,8 @@ class Launch(C.Structure):
                 ("flags", C.c_int32)]
 
 
+NOALLOC_MAX_ELECTRONS = 1 << 20
+
 LAUNCH_INLINE_GUARD = 1
 LAUNCH_DEFER_GUARD = 2
 LAUNCH_SERVICE_GUARD = 4

@@ -59,6 +59,9 @@ def lib():
     L.mcd_forward.restype = i32
     L.mcd_forward.argtypes = [vp, P(_abi.Config), vp, u64, u32, u64, u64, vp, P(_abi.ScatterBuf), vp,
                               P(_abi.Launch), vp]
+    L.mcd_forward_noalloc.restype = i32
+    L.mcd_forward_noalloc.argtypes = [vp, P(_abi.Config), vp, u64, u32, u64, u64, vp, P(_abi.ScatterBuf), vp,
+                                      P(_abi.Launch), vp]
     L.mcd_batch_create.restype = i32
     L.mcd_batch_create.argtypes = [vp, P(_abi.BatchItem), i32, i32, P(vp)]
     L.mcd_batch_run.restype = i32
