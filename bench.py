@@ -66,6 +66,10 @@ def parse():
                     help="instead of the headline line, run the precision study (FP32 vs the all-FP64 GPU run of "
                          "the same streams; DS Cu 20 keV and continuous Au 30 keV x 3 Bethe laws x 2 tilts) and "
                          "print one JSON line per workload")
+    ap.add_argument("--fixed-shapes", action="store_true",
+                    help="skip every autotuner: the headline runs served 1024/8 (unless --block/--guard say "
+                         "otherwise), the differentiation forward and the continuous runs the library default, "
+                         "the batch 128 threads, 128 pick nodes, guard inline (for kernel profiles)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed steps, run one device-to-device copy of this size (the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE readings)")
@@ -135,7 +139,7 @@ def cpu_baselines(tr, seconds, seed):
     return c_base, b1
 
 
-def backward_ratio(n, reps=3):
+def backward_ratio(n, reps=3, tune=True):
     """Forward and backward kernel times (HIP events on the launch stream) on
     the differentiation workload, DS and continuous JL; the forward runs at
     the launch shape the autotuner picks for that workload."""
@@ -155,7 +159,7 @@ def backward_ratio(n, reps=3):
     for key, mode in (("DS", "DS"), ("cont_JL", "cont")):
         spec = BackscatterSpec(mode=mode, n=n, seed=4)
         tr = spec.transport(spec.params().tolist())
-        shape = tr.tune(n, seed=4, reps=1)
+        shape = tr.tune(n, seed=4, reps=1) if tune else dict(block=0, waves_per_simd=0, guard=None, refill_period=1)
         tr.launch(n, els, cnt, seed=4)
         tr.vjp(n, g, seed=4, out=acc)
         torch.cuda.synchronize(dev)
@@ -212,7 +216,7 @@ def two_stream(tr, n, steps, seed):
             "note": "consecutive batches overlap: reported beside value, not as value"}
 
 
-def batched_ratio(reps=3):
+def batched_ratio(reps=3, tune=True):
     """64 configurations: 64 sequential mcd_forward launches vs one autotuned,
     graph-replayed batched launch (same Philox keys)."""
     import torch
@@ -243,7 +247,8 @@ def batched_ratio(reps=3):
         return t_best * 1e3
 
     t_seq = best(seq)
-    tuned = autotune.tune_batch(items, seed=3, reps=1, use_cache=False)
+    tuned = (autotune.tune_batch(items, seed=3, reps=1, use_cache=False) if tune else
+             dict(block=128, pick_nodes=128, guard="inline", ms={}))
     b = engine.Batch(items, pick_nodes=tuned["pick_nodes"])
     t_bat = best(lambda: b.launch(seed=3, use_graph=True, block=tuned["block"], guard=tuned["guard"]))
     return {"workload": "64 configurations {Al,Si,Cu,Au} x E0 {5,10,20,30} keV x tilt {0,70} deg x "
@@ -254,7 +259,7 @@ def batched_ratio(reps=3):
                       "candidates_ms": tuned["ms"]}}
 
 
-def config_sweep(steps):
+def config_sweep(steps, tune=True):
     """Throughput of the configurations the parity tests run: the all-FP64
     kernel on Si 10 keV (2,000 electrons, seed 0), the DS headline (Cu 20 keV,
     70 deg, 2^22 electrons, seed 1) and continuous Au 30 keV -> 1 keV with the
@@ -276,7 +281,7 @@ def config_sweep(steps):
     for name, n, seed, kw, fp64 in runs:
         tr = engine.Transport(**kw)
         prec = _abi.PREC_FP64 if fp64 else _abi.PREC_FP32
-        if not fp64:
+        if not fp64 and tune:
             tr.tune(n, seed=seed, reps=1, use_cache=False)
         els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
@@ -353,7 +358,7 @@ def main():
               "but no multi-GPU run is launched", file=sys.stderr)
         sys.exit(2)
     if a.config_sweep:
-        config_sweep(a.steps)
+        config_sweep(a.steps, tune=not a.fixed_shapes)
         return
     if a.valu_ceiling:
         valu_ceiling()
@@ -375,6 +380,8 @@ def main():
     # launch shape (block, waves/SIMD, guard placement) from the autotuner,
     # outside the timed region; an explicit --block/--waves-per-simd skips it
     tuned = None
+    if a.fixed_shapes and not (a.block or a.waves_per_simd or a.guard):
+        a.block, a.waves_per_simd, a.guard = 1024, 8, "service"
     if not (a.block or a.waves_per_simd or a.guard):
         tuned = tr.tune(n, seed=a.seed, use_cache=False)
     for _ in range(a.warmup):
@@ -446,11 +453,11 @@ def main():
     if not a.no_two_stream:
         out["two_stream"] = two_stream(tr, n, a.steps, a.seed)
     if not a.no_backward:
-        bw = backward_ratio(a.bwd_electrons)
+        bw = backward_ratio(a.bwd_electrons, tune=not a.fixed_shapes)
         out["backward"] = bw
         out["bwd_fwd_ratio"] = bw["ratio_max"]
     if not a.no_batched:
-        out["batched"] = batched_ratio()
+        out["batched"] = batched_ratio(tune=not a.fixed_shapes)
     if not a.no_cpu_baseline:
         c_base, b1 = cpu_baselines(tr, a.cpu_seconds, a.seed)
         out["cpu_baseline"] = c_base
