@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "mcd_device.h"
@@ -35,6 +36,7 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 struct mcd_ctx {
   int device;
   int n_cu;
+  int n_xcc;                    // XCDs of the device (each with its own L2)
   unsigned long long* d_queue;  // [0] backward work counter
   std::vector<ParkBuf> parks;   // deferred-guard park buffers, one per stream
   ParkRec* scratch;             // this device's address of `scratch` (mcd_forward_noalloc)
@@ -109,6 +111,9 @@ extern "C" int mcd_ctx_create(int device, mcd_ctx** out) {
   mcd_ctx* c = new mcd_ctx();
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
+  c->n_xcc = 1;
+  if (hipDeviceGetAttribute(&c->n_xcc, hipDeviceAttributeNumberOfXccs, device) != hipSuccess || c->n_xcc < 1)
+    c->n_xcc = 1;
   c->err[0] = 0;
   if (hipMalloc(&c->d_queue, 64 * sizeof(unsigned long long)) != hipSuccess) {
     delete c;
@@ -686,7 +691,9 @@ struct mcd_batch {
   int n_items;
   int slice_floats;
   std::vector<mcd_tables*> own_tables;  // per-item pick tables sized for the LDS slices
-  BatchItemDev* d_items;
+  BatchItemDev* d_items;                // grouped by XCD, followed by the group bounds (n_groups + 1 ints)
+  int* d_group_start;
+  int n_groups;
   unsigned long long* d_work;           // queues [n], active [n], parked [n], resumed [n]
   ParkRec* d_park;                      // deferred guard: parked electrons, n_i per DS item
   bool has_ds;
@@ -737,7 +744,6 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
   std::vector<BatchItemDev> h(n_items);
   std::vector<double> work(n_items);
   int slice = 0;
-  double tot = 0.;
   for (int i = 0; i < n_items; ++i) {
     const mcd_batch_item& it = items[i];
     int rc = check_config(ctx, it.cfg);
@@ -790,17 +796,35 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
     h[i].R.els = it.els;
     work[i] = (double)it.n * work_per_electron(*it.cfg);
     h[i].cost = (float)work_per_electron(*it.cfg);
-    tot += work[i];
   }
-  double acc = 0.;
-  for (int i = 0; i < n_items; ++i) {
-    h[i].cum_lo = (float)(tot > 0 ? acc / tot : 0.);
-    acc += work[i];
-    h[i].cum_hi = (float)(tot > 0 ? acc / tot : 0.);
+  // One group of items per XCD: largest estimated work first, each to the
+  // group with the least work so far (longest-processing-time partition).
+  b->n_groups = std::max(1, std::min(std::min(ctx->n_xcc, kMaxXcdGroups), n_items));
+  std::vector<int> by_work(n_items);
+  for (int i = 0; i < n_items; ++i) by_work[i] = i;
+  std::stable_sort(by_work.begin(), by_work.end(), [&](int a, int c) { return work[a] > work[c]; });
+  std::vector<std::vector<int>> groups(b->n_groups);
+  std::vector<double> load(b->n_groups, 0.);
+  for (int i : by_work) {
+    const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    groups[g].push_back(i);
+    load[g] += work[i];
   }
-  h[n_items - 1].cum_hi = 1.01f;
+  std::vector<int> order, group_start(1, 0);
+  for (int g = 0; g < b->n_groups; ++g) {
+    double acc = 0.;
+    for (int i : groups[g]) {
+      h[i].cum_lo = (float)(load[g] > 0 ? acc / load[g] : 0.);
+      acc += work[i];
+      h[i].cum_hi = (float)(load[g] > 0 ? acc / load[g] : 0.);
+    }
+    if (!groups[g].empty()) h[groups[g].back()].cum_hi = 1.01f;
+    order.insert(order.end(), groups[g].begin(), groups[g].end());
+    group_start.push_back((int)order.size());
+  }
   b->slice_floats = batched_cfg_floats() + (slice > 0 ? slice : 4);
-  if (hipMalloc(&b->d_items, n_items * sizeof(BatchItemDev)) != hipSuccess ||
+  const size_t items_bytes = ((size_t)n_items * sizeof(BatchItemDev) + 255) & ~(size_t)255;
+  if (hipMalloc(&b->d_items, items_bytes + (kMaxXcdGroups + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc(&b->d_work, 4 * (size_t)n_items * sizeof(unsigned long long)) != hipSuccess) {
     mcd_batch_destroy(b);
     return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the batch failed");
@@ -826,7 +850,15 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
       off += items[i].n;
     }
   }
-  hipError_t e = hipMemcpy(b->d_items, h.data(), n_items * sizeof(BatchItemDev), hipMemcpyHostToDevice);
+  std::vector<BatchItemDev> hs(n_items);
+  for (int k = 0; k < n_items; ++k) {
+    hs[k] = h[order[k]];
+    hs[k].slot = order[k];
+  }
+  b->d_group_start = reinterpret_cast<int*>(reinterpret_cast<char*>(b->d_items) + items_bytes);
+  hipError_t e = hipMemcpy(b->d_items, hs.data(), n_items * sizeof(BatchItemDev), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_group_start, group_start.data(), group_start.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) { mcd_batch_destroy(b); return set_err(ctx, MCD_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e)); }
   *out = b;
   return MCD_OK;
@@ -838,17 +870,21 @@ static hipError_t batch_enqueue(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint6
   hipError_t e = launch_fill_u64(b->d_work, 0ull, 4 * b->n_items, s);
   unsigned long long* cnt = (unsigned long long*)counters;
   const bool deferred = form == 1;
+  auto run = [&](int phase, int grid) {
+    return launch_batched(b->d_items, b->n_items, b->d_group_start, b->n_groups, b->slice_floats, seed, cnt, phase,
+                          grid, block, s);
+  };
   if (form == 3) {
     const int g3 = ctx->n_cu * batched_blocks_per_cu(3, block, b->slice_floats);
-    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 3, g3, block, s);
+    if (e == hipSuccess) e = run(3, g3);
   } else if (deferred) {
     const int g1 = ctx->n_cu * batched_blocks_per_cu(1, block, b->slice_floats);
     const int g2 = ctx->n_cu * batched_blocks_per_cu(2, block, b->slice_floats);
-    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 1, g1, block, s);
-    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 2, g2, block, s);
+    if (e == hipSuccess) e = run(1, g1);
+    if (e == hipSuccess) e = run(2, g2);
   } else {
     const int g0 = ctx->n_cu * batched_blocks_per_cu(0, block, b->slice_floats);
-    if (e == hipSuccess) e = launch_batched(b->d_items, b->n_items, b->slice_floats, seed, cnt, 0, g0, block, s);
+    if (e == hipSuccess) e = run(0, g0);
   }
   if (e == hipSuccess) e = launch_count_outcomes_batched(b->d_items, b->n_items, cnt, s);
   return e;

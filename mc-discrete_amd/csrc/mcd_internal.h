@@ -32,17 +32,24 @@ struct RunArgs {
   uint32_t refill_mask;              // idle lanes are refilled every (refill_mask + 1) steps
 };
 
-// One configuration of a batched launch (device-resident).
+// One configuration of a batched launch (device-resident).  The device array
+// holds the items grouped by XCD (group g = items [group_start[g],
+// group_start[g + 1]), largest estimated work first within a group).
 struct BatchItemDev {
   DevCfg c;
   RunArgs R;                   // els, n, queue (per item); seed/counters given at launch
   float cost;                  // estimated work per electron (load balancing)
-  float cum_lo, cum_hi;        // work-fraction interval for the initial wave assignment
+  float cum_lo, cum_hi;        // the item's share of its group's work: initial wavefront placement
+  int slot;                    // the item's index in the caller's list: its counter block
   unsigned long long* active;  // wavefronts currently working on the item
 };
 
-hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
-                          unsigned long long* counters, int phase, int grid, int block, hipStream_t s);
+// At most this many XCD groups (MI355X has 8 XCDs).
+constexpr int kMaxXcdGroups = 16;
+
+hipError_t launch_batched(const BatchItemDev* items, int n_items, const int* group_start, int n_groups,
+                          int slice_floats, uint64_t seed, unsigned long long* counters, int phase, int grid,
+                          int block, hipStream_t s);
 int batched_blocks_per_cu(int phase, int block, int slice_floats);
 int batched_cfg_floats();
 

@@ -561,11 +561,13 @@ __device__ __forceinline__ void item_work(const BatchItemDev& it, unsigned long 
   }
 }
 
+// The item with the most estimated work left per wavefront already on it,
+// among items [lo, hi); -1 when none has electrons left to claim.
 template <int PHASE>
-__device__ __forceinline__ int next_item(const BatchItemDev* __restrict__ items, int n_items, int lane) {
+__device__ __forceinline__ int most_work_left(const BatchItemDev* __restrict__ items, int lo, int hi, int lane) {
   float best = 0.f;
   int bi = -1;
-  for (int c = lane; c < n_items; c += 64) {
+  for (int c = lo + lane; c < hi; c += 64) {
     unsigned long long q, n;
     item_work<PHASE>(items[c], q, n);
     if (q < n) {
@@ -580,6 +582,22 @@ __device__ __forceinline__ int next_item(const BatchItemDev* __restrict__ items,
     if (ob > best || (ob == best && oi >= 0 && (bi < 0 || oi < bi))) { best = ob; bi = oi; }
   }
   return __builtin_amdgcn_readfirstlane(bi);
+}
+
+// The next item for a wavefront of XCD group [lo, hi): one of the group's
+// items while any has electrons left to claim, so that the XCD's L2 holds the
+// tables of its own group only; after that, help elsewhere.
+template <int PHASE>
+__device__ __forceinline__ int next_item(const BatchItemDev* __restrict__ items, int n_items, int lo, int hi,
+                                         int lane) {
+  const int ci = most_work_left<PHASE>(items, lo, hi, lane);
+  return ci >= 0 ? ci : most_work_left<PHASE>(items, 0, n_items, lane);
+}
+
+// The XCD (accelerator complex die) the wavefront runs on: HW_REG_XCC_ID
+// bits [3:0] (hardware register 20 of the gfx94x/gfx950 family).
+__device__ __forceinline__ int xcc_id() {
+  return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u);
 }
 
 // The wavefront's LDS slice holds a copy of the configuration constants
@@ -665,8 +683,8 @@ static size_t batched_lds_bytes(int phase, int block, int slice_floats) {
 // phase), each faster than the compiler's own allocation despite small spills.
 template <int PHASE>
 __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ? 5 : 4) void k_transport_batched(
-    const BatchItemDev* __restrict__ items, int n_items, int slice_floats, uint64_t seed,
-    unsigned long long* __restrict__ counters) {
+    const BatchItemDev* __restrict__ items, int n_items, const int* __restrict__ group_start, int n_groups,
+    int slice_floats, uint64_t seed, unsigned long long* __restrict__ counters) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -694,16 +712,25 @@ __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ?
     }
   }
   GuardBox* box = PHASE == 3 ? boxes + (wib - 1) : nullptr;
+  const int grp = xcc_id() % n_groups;
+  const int glo = __builtin_amdgcn_readfirstlane(group_start[grp]);
+  const int ghi = __builtin_amdgcn_readfirstlane(group_start[grp + 1]);
   int ci = -1;
   if (PHASE != 2) {
-    const int waves_total = gridDim.x * (blockDim.x >> 6);
-    const int wave_id = blockIdx.x * (blockDim.x >> 6) + wib;
-    const float f = ((float)wave_id + 0.5f) / (float)waves_total;
-    for (int c = 0; c < n_items; ++c)
+    // spread the group's wavefronts over its items in proportion to their
+    // estimated work, so a wavefront mostly stays on one configuration
+    // (blocks are placed on the XCDs round-robin: this XCD holds every
+    // n_groups-th block)
+    const int wpb = (int)(blockDim.x >> 6);
+    const int local = (int)(blockIdx.x / n_groups) * wpb + wib;
+    const int total = (int)((gridDim.x + n_groups - 1) / n_groups) * wpb;
+    const float f = ((float)local + 0.5f) / (float)total;
+    for (int c = glo; c < ghi; ++c)
       if (f >= items[c].cum_lo && f < items[c].cum_hi) ci = c;
     ci = __builtin_amdgcn_readfirstlane(ci);
   }
-  if (ci < 0 || __atomic_load_n(items[ci].R.queue, __ATOMIC_RELAXED) >= items[ci].R.n) ci = next_item<PHASE>(items, n_items, lane);
+  if (ci < 0 || __atomic_load_n(items[ci].R.queue, __ATOMIC_RELAXED) >= items[ci].R.n)
+    ci = next_item<PHASE>(items, n_items, glo, ghi, lane);
   while (ci >= 0) {
     const BatchItemDev& it = items[ci];
     if (lane == 0) atomicAdd(it.active, 1ull);
@@ -740,11 +767,11 @@ __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ?
       }
     }
     const uint64_t cnt[2] = {ne, nf};
-    unsigned long long* ic = counters + (size_t)ci * MCD_NCOUNTERS;
+    unsigned long long* ic = counters + (size_t)it.slot * MCD_NCOUNTERS;
     wave_add_counters(ic + MCD_CNT_EVENTS, cnt, 1, lane);
     wave_add_counters(ic + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
     if (lane == 0) atomicSub(it.active, 1ull);
-    ci = next_item<PHASE>(items, n_items, lane);
+    ci = next_item<PHASE>(items, n_items, glo, ghi, lane);
     __builtin_amdgcn_wave_barrier();
   }
   if constexpr (PHASE == 3) {
@@ -758,22 +785,24 @@ static const void* batched_kernel() {
   return reinterpret_cast<const void*>(&k_transport_batched<PHASE>);
 }
 
-hipError_t launch_batched(const BatchItemDev* items, int n_items, int slice_floats, uint64_t seed,
-                          unsigned long long* counters, int phase, int grid, int block, hipStream_t s) {
+hipError_t launch_batched(const BatchItemDev* items, int n_items, const int* group_start, int n_groups,
+                          int slice_floats, uint64_t seed, unsigned long long* counters, int phase, int grid,
+                          int block, hipStream_t s) {
   if (phase == 3 && (block < 128 || block > 1024)) return hipErrorInvalidValue;
+  if (n_groups < 1 || n_groups > kMaxXcdGroups) return hipErrorInvalidValue;
   const size_t lds = batched_lds_bytes(phase, block, slice_floats);
   if (phase == 1)
-    hipLaunchKernelGGL(k_transport_batched<1>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
-                       counters);
+    hipLaunchKernelGGL(k_transport_batched<1>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
+                       n_groups, slice_floats, seed, counters);
   else if (phase == 2)
-    hipLaunchKernelGGL(k_transport_batched<2>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
-                       counters);
+    hipLaunchKernelGGL(k_transport_batched<2>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
+                       n_groups, slice_floats, seed, counters);
   else if (phase == 3)
-    hipLaunchKernelGGL(k_transport_batched<3>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
-                       counters);
+    hipLaunchKernelGGL(k_transport_batched<3>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
+                       n_groups, slice_floats, seed, counters);
   else
-    hipLaunchKernelGGL(k_transport_batched<0>, dim3(grid), dim3(block), lds, s, items, n_items, slice_floats, seed,
-                       counters);
+    hipLaunchKernelGGL(k_transport_batched<0>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
+                       n_groups, slice_floats, seed, counters);
   return hipGetLastError();
 }
 
@@ -792,7 +821,7 @@ __global__ __launch_bounds__(256) void k_count_outcomes_batched(const BatchItemD
   const BatchItemDev& it = items[blockIdx.y];
   count_outcome_words(it.R.els, it.R.n, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x,
                       (uint64_t)gridDim.x * blockDim.x, part,
-                      counters + (size_t)blockIdx.y * MCD_NCOUNTERS + MCD_CNT_ELECTRONS);
+                      counters + (size_t)it.slot * MCD_NCOUNTERS + MCD_CNT_ELECTRONS);
 }
 
 hipError_t launch_count_outcomes_batched(const BatchItemDev* items, int n_items, unsigned long long* counters,
