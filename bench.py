@@ -412,11 +412,22 @@ def main():
     value = events / wall
     ev_per_launch = events / a.steps
     achieved = ev_per_launch * F_OPS_CU_DS / (kern_ms * 1e-3) / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "transport_ds_cu20_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    # per-launch HBM bytes and VALU instructions per event of this kernel and
+    # workload, from its committed rocprofv3 counter runs
+    prof = {}
+    ppath = os.path.join(ROOT, "profiles", "transport_ds_cu20_counters.json")
+    if os.path.exists(ppath):
+        with open(ppath) as f:
+            prof = json.load(f)
+    traffic = prof.get("hbm_bytes_per_launch")
+    issue = None
+    if prof.get("valu_insts_per_event"):
+        rate = prof["valu_insts_per_event"] * ev_per_launch / (kern_ms * 1e-3)
+        issue = {"valu_insts_per_event": prof["valu_insts_per_event"], "wave_insts_per_s": rate,
+                 "ceiling_wave_insts_per_s": prof["valu_ceiling_wave_insts_per_s"],
+                 "frac": rate / prof["valu_ceiling_wave_insts_per_s"],
+                 "source": "VALU instructions per event: profiles/transport_ds_cu20_counters.json (rocprofv3 "
+                           "SQ_INSTS_VALU); ceiling: v_fma_f32 issue rate measured by --valu-ceiling"}
     out = {
         "metric": METRIC,
         "value": value,
@@ -438,12 +449,14 @@ def main():
         "backscatter_yield": float(c[_abi.CNT_BKS]) / (n * a.steps),
         "fp64_guard_picks_per_event": float(c[_abi.CNT_FP64_PICKS]) / max(events, 1),
         "kernel_ms": kern_ms,
-        # compute roof: FP32 157.3 TFLOP/s (the packed-FP32 VALU rate, equal to the
-        # dense FP32 MFMA figure); the transport's arithmetic is vector FP32
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        # the transport is bound by VALU issue (no MFMA, 0.06 ms of HBM time per
+        # launch): algorithmic FP32 operations against the published FP32 peak,
+        # and the measured issue rate against the measured VALU issue ceiling
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "pipe": "VALU FP32", "ops_per_event": F_OPS_CU_DS,
-                     "frac_vs_nonpacked_78_6": achieved / 78.6},
+                     "traffic_source": "FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated "
+                                       "(profiles/transport_ds_cu20_counters.json)",
+                     "pipe": "VALU (FP32 and integer)", "ops_per_event": F_OPS_CU_DS, "issue": issue},
         "cpu_baseline": None,
         "launch_shape": ({"block": tuned["block"], "waves_per_simd": tuned["waves_per_simd"], "guard": tuned["guard"],
                           "refill_period": tuned.get("refill_period", 1),
