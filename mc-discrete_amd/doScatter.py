@@ -4,15 +4,14 @@
 Reads the input file (fileTools.readInput), builds the tables, runs
 inputPar['num_el'] trajectories on the GPU in chunks and writes the ``input``,
 ``els`` and ``scats`` tables.  The store is named as the reference names it
-(doScatter.py:175-186) under --out (default ./data).  With PyTables importable
-the store is an HDF5 file written through pandas.HDFStore(format='table',
-data_columns=True) like the reference's listener (doScatter.py:112-117);
-otherwise the same tables are written as Parquet files in a directory of that
-name.  -u (units) is accepted and ignored, as it is unused in the reference.
+(doScatter.py:175-186) under --out (default ./data).  The store is an HDF5
+file: written through pandas.HDFStore(format='table', data_columns=True) like
+the reference's listener (doScatter.py:112-117) when PyTables is importable,
+otherwise by mcd.hdf5 with the same groups, tables and attributes.  -u
+(units) is accepted and ignored, as it is unused in the reference.
 """
 from __future__ import annotations
 
-import json
 import os
 import sys
 import time
@@ -58,7 +57,12 @@ def store_name(p):
 
 
 class Sink:
-    """Appends column dicts to an HDF5 store (PyTables present) or Parquet."""
+    """Appends column dicts to the run's HDF5 store: through pandas.HDFStore
+    (format='table', data_columns=True) when PyTables is importable, as the
+    reference's listener does (doScatter.py:112-117); otherwise through
+    mcd.hdf5.H5Store, which writes the same groups and tables itself."""
+
+    MIN_ITEMSIZE = {"els": {"outcome": 6}, "scats": {"type": 12}}
 
     def __init__(self, path, inputPar):
         import pandas as pd
@@ -71,29 +75,32 @@ class Sink:
             self.hdf = True
         except ImportError:
             self.hdf = False
+        if os.path.exists(path):
+            os.remove(path)
         if self.hdf:
-            if os.path.exists(path):
-                os.remove(path)
             with pd.HDFStore(path, mode="w") as st:
-                st["input"] = pd.Series(list(inputPar.values()), index=list(inputPar.keys()), dtype=str)
+                st["input"] = pd.DataFrame(pd.Series(inputPar)).T   # doScatter.py:66
+            self.store = None
         else:
-            os.makedirs(path, exist_ok=True)
-            with open(os.path.join(path, "input.json"), "w") as f:
-                json.dump({k: (v if isinstance(v, (int, float, str, list, bool)) else str(v))
-                           for k, v in inputPar.items()}, f, indent=1)
-            self.parts = {"els": 0, "scats": 0}
+            from mcd.hdf5 import H5Store
+
+            self.store = H5Store(path)
+            self.store.put_input(inputPar)
 
     def append(self, key, cols):
         if cols is None or not cols:
             return
-        df = self.pd.DataFrame(cols)
         if self.hdf:
+            df = self.pd.DataFrame(cols)
             with self.pd.HDFStore(self.path) as st:
                 st.put(key, df, format="table", data_columns=True, append=True,
-                       min_itemsize={"outcome": 6} if key == "els" else {"type": 12})
+                       min_itemsize=self.MIN_ITEMSIZE.get(key))
         else:
-            df.to_parquet(os.path.join(self.path, f"{key}.{self.parts[key]:05d}.parquet"))
-            self.parts[key] += 1
+            self.store.append(key, cols, min_itemsize=self.MIN_ITEMSIZE.get(key))
+
+    def close(self):
+        if self.store is not None:
+            self.store.close()
 
 
 def main(argv=None):
@@ -126,6 +133,7 @@ def main(argv=None):
         done += n
         events += r.events
         print(f"   {done}/{p['num_el']} trajectories, {events} scatter events", flush=True)
+    sink.close()
     print(f" BSE data has been written to {sink.path} ({time.time() - t0:.1f} s)")
     return sink.path
 

@@ -167,8 +167,48 @@ def test_multitraj_cont_matches_oracle():
     compare_columns(els, scats, want_els, want_scats, table_losses=False)
 
 
-def test_doscatter_cli_writes_tables(tmp_path):
+def read_store(path, key):
+    """A table of the CLI's HDF5 store: pandas.read_hdf where PyTables is
+    importable, else the store module's own reader."""
     import pandas as pd
+    try:
+        import tables  # noqa: F401
+        return pd.read_hdf(path, key)
+    except ImportError:
+        from mcd.hdf5 import read_frame
+        return read_frame(path, key)
+
+
+def test_doscatter_cli_output_matches_oracle(tmp_path):
+    """doScatter.py end to end (input file -> tables -> chunked GPU launches
+    -> HDF5 store) against the oracle's records and per-step trace of the same
+    electrons: Si 10 keV -> 1 keV, 70 deg, Wc 100 eV, tol 1e-2, seed 0,
+    200 electrons in chunks of 64, every els and scats column the reference's
+    listener writes (doScatter.py:100-117, fileTools.py:67-99, 193-242)."""
+    import doScatter
+    from mcd.fileTools import readInput
+    from mcd.material import Material
+    from mcd.multiScatter import genTables
+    f = tmp_path / "in.file"
+    f.write_text("num_el : 200\nE0 : 10000\nEmin : 1000\nmaterial : Si\ns_tilt : 70\nelastic : Ruth_vanilla\n"
+                 "mode : DS\nBethe_model : classical\ndiffMFP : no\ngen_tables : yes\ntol_E : 1e-2\ntol_W : 1e-2\n"
+                 "maxScatt : off\nWc : 100\nseed : 0\nelectron_output : outcome, final_E, final_dir, last_pos\n"
+                 "scatter_output : type, E, E_loss, pol_angle, az_angle, pathl, position, energy\n")
+    path = doScatter.main(["-i", str(f), "-o", str(tmp_path / "out"), "-c", "64"])
+    els, scats = read_store(path, "els"), read_store(path, "scats")
+    inp = read_store(path, "input")
+    assert inp.shape[0] == 1 and str(inp["material"].iloc[0]) == "Si" and int(float(inp["num_el"].iloc[0])) == 200
+    p = readInput(str(f))
+    tr = genTables(p).transport(p)
+    gm, gg = tr.grids
+    o, trace, _ = O.run(tr.cfg, O.Table(gm.Es, gm.Ws, tr.cdf(0)), O.Table(gg.Es, gg.Ws, tr.cdf(1)), 0, 0, 200,
+                        trace_cap=400000)
+    want_els, want_scats = oracle_columns(o, trace, Material("Si"))
+    compare_columns({k: els[k].to_numpy() for k in els.columns}, {k: scats[k].to_numpy() for k in scats.columns},
+                    want_els, want_scats)
+
+
+def test_doscatter_cli_writes_tables(tmp_path):
     import doScatter
     f = tmp_path / "in.file"
     f.write_text("num_el : 3000\nE0 : 10000\nEmin : 1000\nmaterial : Si\ns_tilt : 70\nelastic : Ruth_vanilla\n"
@@ -176,14 +216,8 @@ def test_doscatter_cli_writes_tables(tmp_path):
                  "maxScatt : off\nWc : 100\nseed : 0\nelectron_output : outcome, final_E, final_dir, last_pos\n"
                  "scatter_output : type, E, energy\n")
     path = doScatter.main(["-i", str(f), "-o", str(tmp_path / "out"), "-c", "1000"])
-    if path.endswith(".h5") and os.path.isfile(path):
-        els = pd.read_hdf(path, "els")
-        scats = pd.read_hdf(path, "scats")
-    else:
-        els = pd.concat([pd.read_parquet(os.path.join(path, f)) for f in sorted(os.listdir(path))
-                         if f.startswith("els")])
-        scats = pd.concat([pd.read_parquet(os.path.join(path, f)) for f in sorted(os.listdir(path))
-                           if f.startswith("scats")])
+    assert path.endswith(".h5") and os.path.isfile(path)
+    els, scats = read_store(path, "els"), read_store(path, "scats")
     assert len(els) >= 3000 and list(els.columns[:2]) == ["outcome", "final_E"]
     assert set(els["outcome"]) <= {"abs", "bks", "trsm"}
     assert list(scats.columns) == ["type", "E", "energy"]
@@ -194,7 +228,6 @@ def test_doscatter_cli_writes_tables(tmp_path):
 def test_doscatter_without_scatter_rows_pipelines_launches(tmp_path):
     """With no scatter_output, doScatter runs its chunks as launches pipelined
     over two streams; the els table equals one launch over all electrons."""
-    import pandas as pd
     import doScatter
     from mcd.multiScatter import genTables, run_columns
     f = tmp_path / "in.file"
@@ -203,11 +236,7 @@ def test_doscatter_without_scatter_rows_pipelines_launches(tmp_path):
             "maxScatt : off\nWc : 100\nseed : 0\nelectron_output : outcome, final_E, final_dir, last_pos\n")
     f.write_text(text)
     path = doScatter.main(["-i", str(f), "-o", str(tmp_path / "out"), "-c", "1000"])
-    if path.endswith(".h5") and os.path.isfile(path):
-        els = pd.read_hdf(path, "els")
-    else:
-        els = pd.concat([pd.read_parquet(os.path.join(path, f)) for f in sorted(os.listdir(path))
-                         if f.startswith("els")], ignore_index=True)
+    els = read_store(path, "els")
     from mcd.fileTools import readInput
     p = readInput(str(f))
     one, _, _ = run_columns(p, 2500, tables=genTables(p), first=0)
