@@ -261,6 +261,40 @@ int mcd_tables_info(const mcd_tables* t, int* n_nodes, int* n_swap_cells, int* n
  * the same count.  Waits for the device before freeing the old table. */
 int mcd_tables_set_pick_nodes(mcd_ctx* ctx, mcd_tables* t, int n_nodes);
 
+/* Tolerance-driven table grids (probTable.findWseries / findEseries,
+ * MC/tools/probTables.py:280-382): the float32 W grid geomspace(Wmin,
+ * (Emax - Ef)/2, nW) and E grid over [Emin, Emax] the reference's search
+ * picks at tol_W / tol_E, with an FP64 adaptive Gauss-Kronrod integral in
+ * place of scipy.integrate.quad.  Host code only (no device, no context).
+ * kind: MCD_TABLE_MOLLER (Wmin = Wc, n_el = n_val, Eb = 0) or MCD_TABLE_GRYZ
+ * (Wmin = Eb, n_el = ns of the shell; Emin raised to Eb when below it).
+ * nE / nW: capacities of Es / Ws on entry, grid sizes on return; Es = Ws =
+ * NULL returns the sizes only. */
+int mcd_grid_search(int kind, double Wmin, double n_el, double Eb, double Ef, double Emin, double Emax,
+                    double tol_E, double tol_W, float* Es, int32_t* nE, float* Ws, int32_t* nW);
+/* Tables of a configuration from its input-file parameters (probTables.genTables,
+ * MC/tools/probTables.py:541-614): the grid search above for the Moller table
+ * and the outermost shell's Gryzinski table (the one every Gryzinski event
+ * reads, scattering.py:625), then the CDF rows filled on the device in FP64
+ * (precision MCD_PREC_FP64) or FP32.  Continuous mode: tables without
+ * energy-loss tables, as mcd_tables_create with NULL grids. */
+int mcd_tables_build(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, int precision,
+                     mcd_tables** out);
+/* The grid of one table (MCD_TABLE_MOLLER / MCD_TABLE_GRYZ): sizes in nE, nW;
+ * Es and Ws (capacity nE / nW floats), when not NULL, receive the grid. */
+int mcd_tables_grid(mcd_ctx* ctx, const mcd_tables* t, int which, int32_t* nE, int32_t* nW, float* Es, float* Ws);
+/* Table files (the reference's '{type}_{shell}.table' memmaps,
+ * probTables.py:489-535, whose names omit the parameters they were built
+ * with): one file holds both tables with the parameters they depend on
+ * (Z, E0, Emin, Wc, n_val, E_F, the outer shell's E_b and n_s, tol_E, tol_W,
+ * precision), their grids and CDFs.  mcd_tables_save takes tables from
+ * mcd_tables_build or mcd_tables_load; mcd_tables_load refuses a file built
+ * for other parameters (MCD_ERR_TABLE) and otherwise gives the saved tables
+ * bit for bit. */
+int mcd_tables_save(mcd_ctx* ctx, const mcd_tables* t, const char* path);
+int mcd_tables_load(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, int precision,
+                    const char* path, mcd_tables** out);
+
 /* Run electrons [first, first+n) of one configuration.  els: device array of
  * n records (index = electron - first).  scats may be NULL.  counters: device
  * array of MCD_NCOUNTERS uint64, accumulated into (not cleared).  Philox key =

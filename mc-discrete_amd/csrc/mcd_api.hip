@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "mcd_device.h"
@@ -56,6 +57,11 @@ struct mcd_tables {
   int device;
   mcd_config cfg;
   bool has_w;
+  // search tolerances and CDF precision, known for tables from
+  // mcd_tables_build / mcd_tables_load (keyed = true): what a table file records
+  bool keyed;
+  double tol_E, tol_W;
+  int precision;
   TableDev m, g;
   float* nodes;
   int n_nodes, stride, n_ch;
@@ -488,6 +494,211 @@ extern "C" int mcd_tables_download(mcd_ctx* ctx, const mcd_tables* t, int which,
   if (e == hipSuccess) e = hipMemcpy(cdf_host, tmp, n * sizeof(float), hipMemcpyDeviceToHost);
   (void)hipFree(tmp);
   HIP_TRY(ctx, e);
+  return MCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tables from input-file parameters: the tolerance-driven grid search
+// (mcd_grid.hip) and the device CDF fill (probTables.genTables, :541-614).
+static int build_grids(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, std::vector<float>* E,
+                       std::vector<float>* W, mcd_table_grid* g) {
+  const mcd_material& m = cfg->mat;
+  char err[256];
+  // Moller: losses over [Wc, W_max(E0)], energies over [Emin, E0] (probTables.py:557-575)
+  int rc = search_grid(MCD_TABLE_MOLLER, cfg->Wc, m.n_val, 0.0, m.fermi_e, cfg->Emin, cfg->E0, tol_E, tol_W, E[0],
+                       W[0], err, sizeof err);
+  if (rc) return set_err(ctx, rc, "Moller grid search: %s", err);
+  // Gryzinski: the outermost shell, the table every Gryzinski event reads
+  // (probTables.py:579, scattering.py:625); energies from max(Emin, E_b)
+  const int sh = m.n_shells - 1;
+  const double Eb = m.shell_Eb[sh], ns = m.shell_n[sh];
+  const double lo = cfg->Emin < Eb ? Eb : cfg->Emin;
+  rc = search_grid(MCD_TABLE_GRYZ, Eb, ns, Eb, m.fermi_e, lo, cfg->E0, tol_E, tol_W, E[1], W[1], err, sizeof err);
+  if (rc) return set_err(ctx, rc, "Gryzinski grid search: %s", err);
+  g[0] = {(int32_t)E[0].size(), (int32_t)W[0].size(), E[0].data(), W[0].data(), cfg->Wc, m.n_val, 0.0,
+          MCD_TABLE_MOLLER};
+  g[1] = {(int32_t)E[1].size(), (int32_t)W[1].size(), E[1].data(), W[1].data(), Eb, ns, Eb, MCD_TABLE_GRYZ};
+  return MCD_OK;
+}
+
+extern "C" int mcd_grid_search(int kind, double Wmin, double n_el, double Eb, double Ef, double Emin, double Emax,
+                               double tol_E, double tol_W, float* Es, int32_t* nE, float* Ws, int32_t* nW) {
+  if (!nE || !nW) return set_err(nullptr, MCD_ERR_INVALID, "mcd_grid_search: NULL size argument");
+  if (kind != MCD_TABLE_MOLLER && kind != MCD_TABLE_GRYZ)
+    return set_err(nullptr, MCD_ERR_INVALID, "unknown table kind %d", kind);
+  if (!(tol_E > 0.0 && tol_E < 1.0) || !(tol_W > 0.0 && tol_W < 1.0) || !(Wmin > 0.0) || !(Emax > Emin) ||
+      !(Emin > 0.0))
+    return set_err(nullptr, MCD_ERR_INVALID, "grid search: invalid range or tolerance");
+  std::vector<float> E, W;
+  char err[256];
+  int rc = search_grid(kind, Wmin, n_el, Eb, Ef, Emin, Emax, tol_E, tol_W, E, W, err, sizeof err);
+  if (rc) return set_err(nullptr, rc, "%s", err);
+  const int32_t capE = *nE, capW = *nW;
+  *nE = (int32_t)E.size();
+  *nW = (int32_t)W.size();
+  if (!Es && !Ws) return MCD_OK;
+  if (!Es || !Ws || capE < *nE || capW < *nW)
+    return set_err(nullptr, MCD_ERR_INVALID, "grid buffers hold %d and %d values, the grids need %d and %d", capE,
+                   capW, *nE, *nW);
+  memcpy(Es, E.data(), E.size() * sizeof(float));
+  memcpy(Ws, W.data(), W.size() * sizeof(float));
+  return MCD_OK;
+}
+
+extern "C" int mcd_tables_build(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, int precision,
+                                mcd_tables** out) {
+  if (!ctx || !cfg || !out) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_build: NULL argument");
+  *out = nullptr;
+  if (!(tol_E > 0.0 && tol_E < 1.0) || !(tol_W > 0.0 && tol_W < 1.0))
+    return set_err(ctx, MCD_ERR_INVALID, "tolerances tol_E = %g, tol_W = %g outside (0, 1)", tol_E, tol_W);
+  int rc = check_config(ctx, cfg);
+  if (rc) return rc;
+  if (cfg->mode != MCD_MODE_DS) {
+    rc = mcd_tables_create(ctx, cfg, nullptr, nullptr, precision, out);
+  } else {
+    std::vector<float> E[2], W[2];
+    mcd_table_grid g[2];
+    if ((rc = build_grids(ctx, cfg, tol_E, tol_W, E, W, g))) return rc;
+    rc = mcd_tables_create(ctx, cfg, &g[0], &g[1], precision, out);
+  }
+  if (rc) return rc;
+  (*out)->keyed = true;
+  (*out)->tol_E = tol_E;
+  (*out)->tol_W = tol_W;
+  (*out)->precision = precision;
+  return MCD_OK;
+}
+
+extern "C" int mcd_tables_grid(mcd_ctx* ctx, const mcd_tables* t, int which, int32_t* nE, int32_t* nW, float* Es,
+                               float* Ws) {
+  if (!ctx || !t || !nE || !nW) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_grid: NULL argument");
+  if (!t->has_w) return set_err(ctx, MCD_ERR_INVALID, "tables hold no energy-loss tables (continuous mode)");
+  if (which != MCD_TABLE_MOLLER && which != MCD_TABLE_GRYZ)
+    return set_err(ctx, MCD_ERR_INVALID, "unknown table %d", which);
+  const TableDev& td = which == MCD_TABLE_MOLLER ? t->m : t->g;
+  *nE = td.nE;
+  *nW = td.nW;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (Es) HIP_TRY(ctx, hipMemcpy(Es, td.Es, td.nE * sizeof(float), hipMemcpyDeviceToHost));
+  if (Ws) HIP_TRY(ctx, hipMemcpy(Ws, td.Ws, td.nW * sizeof(float), hipMemcpyDeviceToHost));
+  return MCD_OK;
+}
+
+// Table files.  Little-endian: the 8-byte magic, the key (every parameter the
+// tables depend on), then per table nE, nW, Wmin, n_el, Eb, Es, Ws and the
+// (nE x (nW-1)) float32 CDF in the reference layout (masked = 0).
+namespace {
+const char kTableMagic[8] = {'M', 'C', 'D', 'T', 'A', 'B', '0', '1'};
+struct TableKey {
+  int32_t Z, precision;
+  double E0, Emin, Wc, n_val, fermi_e, Eb_outer, ns_outer, tol_E, tol_W;
+};
+TableKey table_key(const mcd_config* c, double tol_E, double tol_W, int precision) {
+  TableKey k;
+  memset(&k, 0, sizeof k);
+  const mcd_material& m = c->mat;
+  k.Z = m.Z;
+  k.precision = precision;
+  k.E0 = c->E0;
+  k.Emin = c->Emin;
+  k.Wc = c->Wc;
+  k.n_val = m.n_val;
+  k.fermi_e = m.fermi_e;
+  k.Eb_outer = m.n_shells > 0 ? m.shell_Eb[m.n_shells - 1] : 0.0;
+  k.ns_outer = m.n_shells > 0 ? m.shell_n[m.n_shells - 1] : 0.0;
+  k.tol_E = tol_E;
+  k.tol_W = tol_W;
+  return k;
+}
+}  // namespace
+
+extern "C" int mcd_tables_save(mcd_ctx* ctx, const mcd_tables* t, const char* path) {
+  if (!ctx || !t || !path) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_save: NULL argument");
+  if (!t->has_w) return set_err(ctx, MCD_ERR_INVALID, "tables hold no energy-loss tables (continuous mode)");
+  if (!t->keyed)
+    return set_err(ctx, MCD_ERR_INVALID, "only tables from mcd_tables_build or mcd_tables_load carry the "
+                                         "tolerances a table file records");
+  const TableKey k = table_key(&t->cfg, t->tol_E, t->tol_W, t->precision);
+  std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return set_err(ctx, MCD_ERR_INVALID, "cannot write %s", tmp.c_str());
+  bool ok = fwrite(kTableMagic, 1, 8, f) == 8 && fwrite(&k, sizeof k, 1, f) == 1;
+  const TableDev* td[2] = {&t->m, &t->g};
+  const double gparam[2][3] = {{t->cfg.Wc, t->moller_n_el, 0.0}, {k.Eb_outer, k.ns_outer, k.Eb_outer}};
+  for (int q = 0; q < 2 && ok; ++q) {
+    const int32_t dims[2] = {td[q]->nE, td[q]->nW};
+    std::vector<float> Es(td[q]->nE), Ws(td[q]->nW), cdf((size_t)td[q]->nE * (td[q]->nW - 1));
+    int32_t a, b;
+    int rc = mcd_tables_grid(ctx, t, q, &a, &b, Es.data(), Ws.data());
+    if (!rc) rc = mcd_tables_download(ctx, t, q, cdf.data(), cdf.size());
+    if (rc) {
+      fclose(f);
+      remove(tmp.c_str());
+      return rc;
+    }
+    ok = fwrite(dims, sizeof dims, 1, f) == 1 && fwrite(gparam[q], sizeof gparam[q], 1, f) == 1 &&
+         fwrite(Es.data(), sizeof(float), Es.size(), f) == Es.size() &&
+         fwrite(Ws.data(), sizeof(float), Ws.size(), f) == Ws.size() &&
+         fwrite(cdf.data(), sizeof(float), cdf.size(), f) == cdf.size();
+  }
+  ok = (fclose(f) == 0) && ok;
+  if (!ok || rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    return set_err(ctx, MCD_ERR_INVALID, "writing the table file %s failed", path);
+  }
+  return MCD_OK;
+}
+
+extern "C" int mcd_tables_load(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, int precision,
+                               const char* path, mcd_tables** out) {
+  if (!ctx || !cfg || !path || !out) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_load: NULL argument");
+  *out = nullptr;
+  int rc = check_config(ctx, cfg);
+  if (rc) return rc;
+  if (cfg->mode != MCD_MODE_DS) return set_err(ctx, MCD_ERR_INVALID, "continuous mode reads no table file");
+  FILE* f = fopen(path, "rb");
+  if (!f) return set_err(ctx, MCD_ERR_TABLE, "no table file %s", path);
+  char magic[8];
+  TableKey k;
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kTableMagic, 8) == 0 && fread(&k, sizeof k, 1, f) == 1;
+  if (!ok) {
+    fclose(f);
+    return set_err(ctx, MCD_ERR_TABLE, "%s is not a table file", path);
+  }
+  const TableKey want = table_key(cfg, tol_E, tol_W, precision);
+  if (memcmp(&k, &want, sizeof k) != 0) {
+    fclose(f);
+    return set_err(ctx, MCD_ERR_TABLE,
+                   "table file %s was built for other parameters (Z %d E0 %g Emin %g Wc %g tol_E %g tol_W %g "
+                   "precision %d; asked Z %d E0 %g Emin %g Wc %g tol_E %g tol_W %g precision %d)",
+                   path, k.Z, k.E0, k.Emin, k.Wc, k.tol_E, k.tol_W, k.precision, want.Z, want.E0, want.Emin, want.Wc,
+                   want.tol_E, want.tol_W, want.precision);
+  }
+  std::vector<float> E[2], W[2], cdf[2];
+  mcd_table_grid g[2];
+  for (int q = 0; q < 2 && ok; ++q) {
+    int32_t dims[2];
+    double gp[3];
+    ok = fread(dims, sizeof dims, 1, f) == 1 && fread(gp, sizeof gp, 1, f) == 1 && dims[0] >= 2 && dims[1] >= 2 &&
+         dims[0] <= (1 << 24) && dims[1] <= 65536;
+    if (!ok) break;
+    E[q].resize(dims[0]);
+    W[q].resize(dims[1]);
+    cdf[q].resize((size_t)dims[0] * (dims[1] - 1));
+    ok = fread(E[q].data(), sizeof(float), E[q].size(), f) == E[q].size() &&
+         fread(W[q].data(), sizeof(float), W[q].size(), f) == W[q].size() &&
+         fread(cdf[q].data(), sizeof(float), cdf[q].size(), f) == cdf[q].size();
+    g[q] = {dims[0], dims[1], E[q].data(), W[q].data(), gp[0], gp[1], gp[2], q};
+  }
+  ok = ok && fgetc(f) == EOF;
+  fclose(f);
+  if (!ok) return set_err(ctx, MCD_ERR_TABLE, "table file %s is truncated or malformed", path);
+  rc = mcd_tables_upload(ctx, cfg, &g[0], cdf[0].data(), &g[1], cdf[1].data(), out);
+  if (rc) return rc;
+  (*out)->keyed = true;
+  (*out)->tol_E = tol_E;
+  (*out)->tol_W = tol_W;
+  (*out)->precision = precision;
   return MCD_OK;
 }
 

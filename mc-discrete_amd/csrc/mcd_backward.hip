@@ -34,9 +34,12 @@
 //    crosses a threshold.  Each contribution adds the central difference of
 //    the two indicators along the energy tangent,
 //      C * ([E_j + dE_j delta] - [E_j - dE_j delta]) / (2 delta),
-//    with delta the parameter step (default 1% of the parameter).  For a
-//    smooth exit-energy density this is the threshold (boundary) term of the
-//    derivative up to O(delta^2); DS exit energies are a discrete comb
+//    with delta the parameter step (default 1% of the parameter).  The replay
+//    therefore continues past Emin while the electron is alive on one side
+//    of that stencil (E + |dE_i delta_i| > Emin): those steps add the
+//    threshold terms of the contributions that exist at theta -+ delta only.
+//    For a smooth exit-energy density this is the threshold (boundary) term
+//    of the derivative up to O(delta^2); DS exit energies are a discrete comb
 //    (E0 - k E_pl minus table grid values), for which it is the
 //    finite-difference quotient at step delta of the edge crossings, the
 //    quantity a finite-difference check at that step measures.
@@ -273,7 +276,11 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     // derivative dC into the bin of E, plus the threshold terms: the alive
     // indicator [E > Emin] and the bin indicator, differenced along the energy
     // tangent with step delta_i (central difference of the indicator).
+    // Past Emin (base indicator 0) a contribution adds only its threshold
+    // terms: the steps where the electron is alive at theta + delta or
+    // theta - delta but not at theta.
     auto contribute = [&](float C, const float* dC) {
+      const bool base = E > Emin;
       const int b = min((int)(E * A.inv_bin_w), A.n_bins - 1);
       float tv[3];   // threshold term of the yield
       int bp[3], bm[3];
@@ -286,14 +293,14 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
         tv[i] = C * A.inv_2delta[i] * (float)((bp[i] >= 0) - (bm[i] >= 0));
       }
       if (JAC) {
-        accv += C;
+        if (base) accv += C;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) accg[i] += dC[i] + tv[i];
+        for (int i = 0; i < 3; ++i) accg[i] += (base ? dC[i] : 0.f) + tv[i];
         float* hb = hist + 4 * b;
-        atomicAdd(&hb[0], C);
+        if (base) atomicAdd(&hb[0], C);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          atomicAdd(&hb[1 + i], dC[i]);
+          if (base) atomicAdd(&hb[1 + i], dC[i]);
           if (bp[i] != bm[i]) {
             const float f = C * A.inv_2delta[i];
             if (bp[i] >= 0) atomicAdd(&hist[4 * bp[i] + 1 + i], f);
@@ -301,7 +308,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
           }
         }
       } else {
-        const float g = gup[0] + gup[1 + b];
+        const float g = base ? gup[0] + gup[1 + b] : 0.f;
         accv += C * g;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -436,7 +443,11 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) dE[i] -= dW[i];
     step++;
-    if (lost || !(E > Emin) || !(z > 0.f) || w < kWMin || step >= kBwdStepCap) {
+    // alive at theta or at one side of the threshold stencil theta +- delta
+    float Ereach = E;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Ereach = fmaxf(Ereach, E + fabsf(dE[i] * A.delta[i]));
+    if (lost || !(Ereach > Emin) || !(E > 0.f) || !(z > 0.f) || w < kWMin || step >= kBwdStepCap) {
       active = false;
       continue;
     }

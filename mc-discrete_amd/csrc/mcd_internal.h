@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "mcd_device.h"
 
 namespace mcd {
@@ -109,5 +111,12 @@ hipError_t launch_pick_check(const DevCfg& c, int n_checks, float* err, int* bad
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, int n, hipStream_t s);
 hipError_t launch_unmask(const float* src, float* dst, size_t n, hipStream_t s);
 hipError_t launch_mask(const float* src, float* dst, size_t n, hipStream_t s);
+
+// Tolerance-driven grid search of one energy-loss table (mcd_grid.hip):
+// float32 Es over [Emin, Emax] and Ws over [Wmin, (Emax - Ef)/2].
+int search_grid(int kind, double Wmin, double n_el, double Eb, double Ef, double Emin, double Emax, double tol_E,
+                double tol_W, std::vector<float>& Es, std::vector<float>& Ws, char* err, size_t err_len);
+// The reference integral of the search: dsigma/dW over [Wmin, (E - Ef)/2] at E.
+double grid_reference_integral(int kind, double Wmin, double n_el, double Eb, double Ef, double E);
 
 }  // namespace mcd

@@ -4,7 +4,9 @@
 Reads the input file (fileTools.readInput), builds the tables, runs
 inputPar['num_el'] trajectories on the GPU in chunks and writes the ``input``,
 ``els`` and ``scats`` tables.  The store is named as the reference names it
-(doScatter.py:175-186) under --out (default ./data).  The store is an HDF5
+(doScatter.py:175-186) under --out (default ./data); DS tables are kept in a
+parameter-keyed table file under <out>/tables (or the input key table_dir)
+and read back when the input says gen_tables: no.  The store is an HDF5
 file: written through pandas.HDFStore(format='table', data_columns=True) like
 the reference's listener (doScatter.py:112-117) when PyTables is importable,
 otherwise by mcd.hdf5 with the same groups, tables and attributes.  -u
@@ -113,8 +115,10 @@ def main(argv=None):
     print(f" - Number of incident electrons: {p['num_el']}\n - Material is: {p['material']}\n"
           f" - Scattering mode is: {p['mode']}\n - Elastic scattering mode is: {p.get('elastic', 'Ruth_vanilla')}")
     t0 = time.time()
-    tables = genTables(p)
     os.makedirs(outdir, exist_ok=True)
+    # DS tables through a parameter-keyed table file: input key table_dir, by
+    # default <output dir>/tables (gen_tables: no reads it back)
+    tables = genTables(p, table_dir=p.get("table_dir") or os.path.join(outdir, "tables"))
     sink = Sink(os.path.join(outdir, store_name(p)), p)
     done, events = 0, 0
     if not p.get("scatter_output"):

@@ -56,6 +56,17 @@ def lib():
     L.mcd_tables_info.argtypes = [vp, P(i32), P(i32), P(i32), P(C.c_float)]
     L.mcd_tables_set_pick_nodes.restype = i32
     L.mcd_tables_set_pick_nodes.argtypes = [vp, vp, i32]
+    f64 = C.c_double
+    L.mcd_grid_search.restype = i32
+    L.mcd_grid_search.argtypes = [i32, f64, f64, f64, f64, f64, f64, f64, f64, vp, P(i32), vp, P(i32)]
+    L.mcd_tables_build.restype = i32
+    L.mcd_tables_build.argtypes = [vp, P(_abi.Config), f64, f64, i32, P(vp)]
+    L.mcd_tables_grid.restype = i32
+    L.mcd_tables_grid.argtypes = [vp, vp, i32, P(i32), P(i32), vp, vp]
+    L.mcd_tables_save.restype = i32
+    L.mcd_tables_save.argtypes = [vp, vp, C.c_char_p]
+    L.mcd_tables_load.restype = i32
+    L.mcd_tables_load.argtypes = [vp, P(_abi.Config), f64, f64, i32, C.c_char_p, P(vp)]
     L.mcd_forward.restype = i32
     L.mcd_forward.argtypes = [vp, P(_abi.Config), vp, u64, u32, u64, u64, vp, P(_abi.ScatterBuf), vp,
                               P(_abi.Launch), vp]

@@ -8,6 +8,7 @@ as numpy structured arrays (``_abi.ELECTRON_DTYPE`` / ``_abi.SCATTER_DTYPE``).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -97,7 +98,12 @@ class Transport:
                  mode: str = "DS", bethe_model: str = "classical", tol_E: float = 1e-3, tol_W: float = 1e-3,
                  thickness: float = float("inf"), max_scatt: int = 0, J: float | None = None,
                  table_precision: int = _abi.PREC_FP64, device: int = 0, grids=None, cdfs=None,
-                 pick_nodes: int = 0):
+                 pick_nodes: int = 0, grid_search: str = "host", table_file: str | None = None):
+        """DS tables: from `grids` (and `cdfs`) when given; else read from
+        `table_file` (mcd_tables_load: refused when the file was built for
+        other parameters); else built with the grid search run in Python
+        (grid_search="host", scipy's quad as the reference integral) or by the
+        library (grid_search="library", mcd_tables_build)."""
         self.mat = material if isinstance(material, Material) else Material(material)
         self.device = device
         self.cfg = _abi.make_config(self.mat, E0, Emin, Wc, s_tilt, thickness, mode, bethe_model, max_scatt, J)
@@ -109,7 +115,20 @@ class Transport:
         self.tuned = None
         self._tables = C.c_void_p()
         ctx = context(device)
-        if self.mode == _abi.MODE_DS:
+        self.table_precision = int(table_precision)
+        if self.mode == _abi.MODE_DS and self.grids is None and cdfs is None and (
+                table_file is not None or grid_search == "library"):
+            if table_file is not None:
+                check(lib().mcd_tables_load(ctx, C.byref(self.cfg), float(tol_E), float(tol_W), int(table_precision),
+                                            os.fsencode(table_file), C.byref(self._tables)), ctx)
+            else:
+                check(lib().mcd_tables_build(ctx, C.byref(self.cfg), float(tol_E), float(tol_W), int(table_precision),
+                                             C.byref(self._tables)), ctx)
+            self.grids = (self._library_grid(_abi.TABLE_MOLLER, moller_grid(self.mat, E0, Emin, Wc)),
+                          self._library_grid(_abi.TABLE_GRYZ, gryz_grid(self.mat, E0, Emin)))
+            if pick_nodes:
+                check(lib().mcd_tables_set_pick_nodes(ctx, self._tables, int(pick_nodes)), ctx)
+        elif self.mode == _abi.MODE_DS:
             if self.grids is None:
                 self.grids = (moller_grid(self.mat, E0, Emin, Wc).build_grids(tol_E, tol_W),
                               gryz_grid(self.mat, E0, Emin).build_grids(tol_E, tol_W))
@@ -139,6 +158,22 @@ class Transport:
             pass
 
     # -- tables ---------------------------------------------------------------
+    def _library_grid(self, which, grid):
+        ctx = context(self.device)
+        nE, nW = C.c_int32(0), C.c_int32(0)
+        check(lib().mcd_tables_grid(ctx, self._tables, int(which), C.byref(nE), C.byref(nW), None, None), ctx)
+        grid.Es = np.zeros(nE.value, dtype=np.float32)
+        grid.Ws = np.zeros(nW.value, dtype=np.float32)
+        check(lib().mcd_tables_grid(ctx, self._tables, int(which), C.byref(nE), C.byref(nW), grid.Es.ctypes.data,
+                                    grid.Ws.ctypes.data), ctx)
+        return grid
+
+    def save_tables(self, path: str):
+        """Write the DS tables to a parameter-keyed table file (mcd_tables_save;
+        tables from grid_search="library" or a table file only)."""
+        ctx = context(self.device)
+        check(lib().mcd_tables_save(ctx, self._tables, os.fsencode(path)), ctx)
+
     def cdf(self, which: int) -> np.ndarray:
         """Device-built CDF in the reference layout (masked = 0)."""
         g = self.grids[which]

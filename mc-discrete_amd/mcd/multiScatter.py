@@ -20,12 +20,14 @@ within a run; the reference instead re-seeded per job with pid*time
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from .engine import Transport
 from .fileTools import els_columns, per_electron_dicts, scats_columns, zipDict
 from .material import Material
-from .tables import gryz_grid, moller_grid
+from .tables import grid_key, gryz_grid, moller_grid
 
 
 class DeviceTables(dict):
@@ -38,6 +40,7 @@ class DeviceTables(dict):
         self.material = material
         self.next_electron = 0
         self._transports = {}
+        self.table_file = None   # the parameter-keyed table file the DS tables were read from or saved to
 
     def claim(self, n):
         first = self.next_electron
@@ -52,13 +55,15 @@ class DeviceTables(dict):
                float(p.get("thickness", float("inf"))), int(p.get("maxScatt", 0) or 0))
         if key not in self._transports:
             mat = _material(material or self.material, p)
-            grids = (self["Moller"], self["Gryz"][0]) if "Moller" in self else None
+            ds = "DS" in mode
+            from_file = ds and self.table_file is not None
+            grids = (self["Moller"], self["Gryz"][0]) if ("Moller" in self and not from_file) else None
             self._transports[key] = Transport(
                 mat, p["E0"], p["Emin"], Wc=p.get("Wc", 50.0), s_tilt=p.get("s_tilt", 0.0),
-                mode="DS" if "DS" in mode else "cont", bethe_model=p.get("Bethe_model", "classical"),
+                mode="DS" if ds else "cont", bethe_model=p.get("Bethe_model", "classical"),
                 tol_E=p.get("tol_E", 1e-3), tol_W=p.get("tol_W", 1e-3),
                 thickness=p.get("thickness", float("inf")), max_scatt=int(p.get("maxScatt", 0) or 0),
-                grids=grids)
+                grids=grids, table_file=self.table_file if from_file else None)
         return self._transports[key]
 
 
@@ -79,15 +84,49 @@ def _check_elastic(inputPar):
         raise NotImplementedError("diffMFP: the diffraction channel is not implemented")
 
 
-def genTables(inputPar):
-    """Tables for a run (probTables.genTables, probTables.py:541-614)."""
+def table_file(inputPar, table_dir, material=None):
+    """The parameter-keyed table file of a DS run under table_dir: its name
+    holds every parameter the tables depend on (the reference's
+    '{type}_{shell}.table' names hold none, probTables.py:160)."""
+    mat = _material(material, inputPar)
+    key = grid_key("tables", mat, inputPar["E0"], inputPar["Emin"], inputPar["Wc"], inputPar.get("tol_E", 1e-3),
+                   inputPar.get("tol_W", 1e-3))
+    return os.path.join(table_dir, key + "_fp64.mcdt")
+
+
+def genTables(inputPar, table_dir=None):
+    """Tables for a run (probTables.genTables, probTables.py:541-614).
+
+    With a table directory (the argument or inputPar['table_dir']) the DS
+    tables go through a parameter-keyed table file there: gen_tables true
+    builds them (library grid search, device CDF fill) and writes the file;
+    gen_tables false reads the file, and builds and writes it when there is
+    none.  A file built for other parameters is refused, never reused.
+    Without a table directory the tables are built in memory."""
     _check_elastic(inputPar)
     mat = Material(inputPar["material"])
     t = DeviceTables(inputPar, mat)
     if "DS" in inputPar.get("mode", "DS"):
         tol_E, tol_W = inputPar.get("tol_E", 1e-3), inputPar.get("tol_W", 1e-3)
-        t["Moller"] = moller_grid(mat, inputPar["E0"], inputPar["Emin"], inputPar["Wc"]).build_grids(tol_E, tol_W)
-        t["Gryz"] = [gryz_grid(mat, inputPar["E0"], inputPar["Emin"]).build_grids(tol_E, tol_W)]
+        table_dir = table_dir or inputPar.get("table_dir")
+        if table_dir is None:
+            t["Moller"] = moller_grid(mat, inputPar["E0"], inputPar["Emin"], inputPar["Wc"]).build_grids(tol_E, tol_W)
+            t["Gryz"] = [gryz_grid(mat, inputPar["E0"], inputPar["Emin"]).build_grids(tol_E, tol_W)]
+            return t
+        path = table_file(inputPar, table_dir, mat)
+        kw = dict(Wc=inputPar["Wc"], tol_E=tol_E, tol_W=tol_W)
+        if inputPar.get("gen_tables", True) or not os.path.exists(path):
+            if not inputPar.get("gen_tables", True):
+                print(f" - no table file {path}: generating the tables")
+            tr = Transport(mat, inputPar["E0"], inputPar["Emin"], grid_search="library", **kw)
+            os.makedirs(table_dir, exist_ok=True)
+            tr.save_tables(path)
+            print(f" - tables written to {path}")
+        else:
+            tr = Transport(mat, inputPar["E0"], inputPar["Emin"], table_file=path, **kw)
+            print(f" - tables read from {path}")
+        t["Moller"], t["Gryz"] = tr.grids[0], [tr.grids[1]]
+        t.table_file = path
     return t
 
 

@@ -158,6 +158,24 @@ class TableGrid:
         self.Es = self.find_E_series(tol_E)
         return self
 
+    def build_grids_library(self, tol_E, tol_W):
+        """The same search run by the library (mcd_grid_search: host C++, an
+        FP64 Gauss-Kronrod integral in place of scipy's quad)."""
+        import ctypes as C
+
+        from ._lib import check, lib
+
+        kind = 0 if self.kind == "Moller" else 1
+        args = (kind, float(self.Wmin), float(self.n_el), float(self.Eb), float(self.Ef), float(self.Emin),
+                float(self.Emax), float(tol_E), float(tol_W))
+        nE, nW = C.c_int32(0), C.c_int32(0)
+        check(lib().mcd_grid_search(*args, None, C.byref(nE), None, C.byref(nW)))
+        Es = np.zeros(nE.value, dtype=np.float32)
+        Ws = np.zeros(nW.value, dtype=np.float32)
+        check(lib().mcd_grid_search(*args, Es.ctypes.data, C.byref(nE), Ws.ctypes.data, C.byref(nW)))
+        self.Es, self.Ws = Es, Ws
+        return self
+
     @property
     def shape(self):
         return (int(self.Es.size), int(self.Ws.size) - 1)

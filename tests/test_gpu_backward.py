@@ -1,15 +1,24 @@
 """Backward (derivative replay) kernel against central finite differences of
 the forward transport with common random numbers, at the differentiation
 workload: Cu 20 keV -> 1 keV, s_tilt 70 deg, DS (Wc 50 eV) and continuous JL,
-Philox seed 4, relative parameter step 1e-2.
+Philox seed 4, relative parameter step 1e-2, at the nominal point and at the
+ends of the parameter ranges (Wc 20 and 190 eV, s_tilt 60 and 80 deg, J -20%
+and +20%, density scale 0.9 and 1.1).
 
 The finite difference is itself a Monte Carlo estimate: its standard error is
 computed from the per-electron paired outcomes, the replay's from 32
-independent electron ranges.  The yield derivative passes at
-|grad - fd| <= max(5% |fd|, 3 sigma_combined); each histogram bin whose
-finite difference is resolved (|fd| > 10 sigma_fd) at max(5% |fd|, 4 sigma)
-(about 50 such bins are compared in all), and the whole histogram at
-chi^2 <= 2 x (number of bins)."""
+independent electron ranges.  Each yield derivative is checked in two steps:
+the finite difference must be resolved (standard error at most 2% of |fd|,
+electron counts chosen for that), then the replay must agree within 5% of
+|fd|.  Histogram bins whose finite difference is resolved (|fd| > 10
+sigma_fd) are compared at max(5% |fd|, 4 sigma), and the whole histogram at
+chi^2 <= 2 x (number of bins).
+
+Wc's upper end is 190 eV, not 200: the reference's tolerance search changes
+the Moller table's shape between 197 and 200.5 eV (1000 x 99 below 197 eV,
+2000 x 89 up to 200.5, 1000 x 89 above), a step of the model that no
+derivative has, and the central difference at 198 or 200 eV straddles it;
+190 eV +- 1% keeps one shape."""
 import numpy as np
 import pytest
 
@@ -26,14 +35,16 @@ def _spec(mode, n=N):
     return BackscatterSpec(mode=mode, n=n, seed=4)
 
 
-def _paired_fd(spec, name, n):
+def _paired_fd(spec, name, n, rel=REL, p0=None):
     """Central difference of (Y, H) per electron with common random numbers:
-    (fd[1 + n_bins], se[1 + n_bins])."""
+    (fd[1 + n_bins], se[1 + n_bins]).  Per chunk the paired differences are
+    accumulated by bin counts: for bin b, d_b = [+ in b] - [- in b], so
+    sum d_b = n+(b) - n-(b) and sum d_b^2 = n+(b) + n-(b) - 2 n_both(b)."""
     import torch
     from mcd import _abi
-    p0 = spec.params().tolist()
+    p0 = spec.params().tolist() if p0 is None else list(p0)
     gi = GRAD_OF[name]
-    h = REL * abs(p0[gi])
+    h = rel * abs(p0[gi])
     pp, pm = list(p0), list(p0)
     pp[gi] += h
     pm[gi] -= h
@@ -47,31 +58,38 @@ def _paired_fd(spec, name, n):
     sum_d = torch.zeros(1 + nb, dtype=torch.float64, device=dev)
     sum_d2 = torch.zeros(1 + nb, dtype=torch.float64, device=dev)
     off = _abi.ELECTRON_DTYPE.fields["outcome"][1]
+    size = _abi.ELECTRON_DTYPE.itemsize
+
+    def decode(e, k):
+        rec = e[: k * size].view(k, size)
+        bks = rec[:, off] == _abi.OUT_BKS
+        E = rec[:, 0:4].contiguous().view(torch.float32).reshape(k).double()
+        b = torch.clamp((E * (nb / spec.E0)).long(), 0, nb - 1)
+        return bks, b
+
     for first in range(0, n, m):
         k = min(m, n - first)
         tp.launch(k, ep, cnt, seed=spec.seed, first=first)
         tm.launch(k, em, cnt, seed=spec.seed, first=first)
-        cols = []
-        for e in (ep, em):
-            rec = e[: k * _abi.ELECTRON_DTYPE.itemsize].view(k, _abi.ELECTRON_DTYPE.itemsize)
-            bks = (rec[:, off] == _abi.OUT_BKS).to(torch.float64)
-            E = rec[:, 0:4].contiguous().view(torch.float32).reshape(k).double()
-            b = torch.clamp((E * (nb / spec.E0)).long(), 0, nb - 1)
-            onehot = torch.zeros((k, 1 + nb), dtype=torch.float64, device=dev)
-            onehot[:, 0] = bks
-            onehot[torch.arange(k, device=dev), 1 + b] += bks
-            cols.append(onehot)
-        d = cols[0] - cols[1]
-        sum_d += d.sum(0)
-        sum_d2 += (d * d).sum(0)
+        bp, hp = decode(ep, k)
+        bm, hm = decode(em, k)
+        np_, nm_ = bp.sum(), bm.sum()
+        both = bp & bm
+        sum_d[0] += (np_ - nm_).double()
+        sum_d2[0] += (np_ + nm_ - 2 * both.sum()).double()
+        cp = torch.bincount(hp[bp], minlength=nb).double()
+        cm = torch.bincount(hm[bm], minlength=nb).double()
+        cb = torch.bincount(hp[both & (hp == hm)], minlength=nb).double()
+        sum_d[1:] += cp - cm
+        sum_d2[1:] += cp + cm - 2 * cb
     mean = sum_d / n
     var = sum_d2 / n - mean * mean
     return (mean / (2 * h)).cpu().numpy(), (torch.sqrt(var / n) / (2 * h)).cpu().numpy()
 
 
-def _replay(spec, n):
+def _replay(spec, n, p=None):
     from mcd.autograd import jacobian
-    p = spec.params().tolist()
+    p = spec.params().tolist() if p is None else list(p)
     rows = []
     for k in range(K):
         v, j = jacobian(spec, p, n=n // K, first=k * (n // K))
@@ -82,25 +100,33 @@ def _replay(spec, n):
 
 _cache = {}
 
+RESOLVED = 0.02   # the finite difference's standard error, as a share of |fd|, before comparing
+BAR = 0.05        # replay against finite difference, as a share of |fd|
 
-def _data(mode, name, n):
-    key = (mode, name, n)
+
+def _data(mode, name, n_fd, n_rep, base):
+    key = (mode, name, n_fd, n_rep, tuple(sorted(base.items())))
     if key not in _cache:
-        spec = _spec(mode, n)
-        fd, fse = _paired_fd(spec, name, n)
-        if (mode, n) not in _cache:
-            _cache[(mode, n)] = _replay(spec, n)
-        _cache[key] = (fd, fse)
-    return _cache[key], _cache[(mode, n)]
+        spec = _spec(mode, n_fd)
+        kw = dict(base)
+        if "J_rel" in kw:
+            kw["J"] = spec.params().tolist()[2] * kw.pop("J_rel")
+        p0 = spec.params(**kw).tolist()
+        fd, fse = _paired_fd(spec, name, n_fd, p0=p0)
+        rep = _replay(spec, n_rep, p=p0)
+        _cache[key] = ((fd, fse), rep)
+    return _cache[key]
 
 
-def _check(mode, name, n):
-    (fd, fse), (m, se) = _data(mode, name, n)
+def _check(mode, name, n_fd, n_rep=None, base=None):
+    (fd, fse), (m, se) = _data(mode, name, n_fd, n_rep or n_fd, base or {})
     col = 1 + GRAD_OF[name]
     g, gse = m[:, col], se[:, col]
     sig = np.sqrt(gse ** 2 + fse ** 2)
-    print(f"{mode} d/d{name}: dY replay {g[0]:.5e} +- {gse[0]:.1e}, fd {fd[0]:.5e} +- {fse[0]:.1e}")
-    assert abs(g[0] - fd[0]) <= max(0.05 * abs(fd[0]), 3 * sig[0]), (g[0], fd[0], sig[0])
+    print(f"{mode} d/d{name} at {base or 'nominal'}: dY replay {g[0]:.5e} +- {gse[0]:.1e}, "
+          f"fd {fd[0]:.5e} +- {fse[0]:.1e} ({(g[0] - fd[0]) / fd[0]:+.2%})")
+    assert fse[0] <= RESOLVED * abs(fd[0]), ("finite difference not resolved", fd[0], fse[0])
+    assert abs(g[0] - fd[0]) <= BAR * abs(fd[0]), (g[0], fd[0], sig[0])
     # bins whose finite difference is resolved
     for b in range(1, fd.size):
         if abs(fd[b]) > 10 * fse[b]:
@@ -110,15 +136,35 @@ def _check(mode, name, n):
     assert chi2 <= 2.0 * live.sum(), (chi2, int(live.sum()))
 
 
-@pytest.mark.parametrize("mode,name", [("DS", "s_tilt"), ("cont", "s_tilt"), ("cont", "J"), ("DS", "density_scale")])
-def test_backward_matches_central_differences(mode, name):
-    _check(mode, name, N)
+@pytest.mark.parametrize("mode,name,n_fd,n_rep,base", [
+    ("DS", "s_tilt", N, N, {}),
+    ("DS", "s_tilt", N, N, {"s_tilt": 60.0}),
+    ("DS", "s_tilt", N, N, {"s_tilt": 80.0}),
+    ("cont", "s_tilt", N, N, {}),
+    ("cont", "s_tilt", N, N, {"s_tilt": 60.0}),
+    ("cont", "s_tilt", N, N, {"s_tilt": 80.0}),
+    ("cont", "J", 4 * N, 4 * N, {}),
+    ("cont", "J", 4 * N, 4 * N, {"J_rel": 0.8}),
+    ("cont", "J", 4 * N, 4 * N, {"J_rel": 1.2}),
+], ids=["DS-tilt70", "DS-tilt60", "DS-tilt80", "cont-tilt70", "cont-tilt60", "cont-tilt80", "cont-J", "cont-J0.8",
+        "cont-J1.2"])
+def test_backward_matches_central_differences(mode, name, n_fd, n_rep, base):
+    _check(mode, name, n_fd, n_rep, base)
 
 
-def test_backward_wc_matches_central_differences():
-    # the Wc derivative is a likelihood-ratio estimate and the finite difference
-    # of a 1% change in Wc is small: four times the electrons
-    _check("DS", "Wc", 4 * N)
+# The Wc derivative is a likelihood-ratio estimate and a 1% change in Wc moves
+# the yield little: 2^31 electrons for the finite difference (2^30 for the
+# replay) resolve it to 1-1.5%.
+@pytest.mark.parametrize("base", [{}, {"Wc": 20.0}, {"Wc": 190.0}], ids=["50eV", "20eV", "190eV"])
+def test_backward_wc_matches_central_differences(base):
+    _check("DS", "Wc", 128 * N, 64 * N, base)
+
+
+# A 1% density change moves the DS yield by about 3e-5: 2^33 electrons for the
+# finite difference (2^32 for the replay) resolve it to about 2%.
+@pytest.mark.parametrize("base", [{}, {"density_scale": 0.9}, {"density_scale": 1.1}], ids=["1.0", "0.9", "1.1"])
+def test_backward_density_matches_central_differences(base):
+    _check("DS", "density_scale", 512 * N, 256 * N, base)
 
 
 @pytest.mark.parametrize("mode", ["DS", "cont"])

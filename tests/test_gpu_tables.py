@@ -1,0 +1,118 @@
+"""Tables built by the library from input-file parameters (mcd_tables_build:
+grid search plus the device CDF fill) and parameter-keyed table files
+(mcd_tables_save / mcd_tables_load), the counterparts of
+probTables.genTables / mapToMemory / readFromMemory (probTables.py:489-614).
+
+Pins: the grids equal the Python search's (scipy quad) and the CDFs equal the
+tables built from those grids bit for bit, hence the float32 restatement of
+computeBlock within 1e-5; a table file gives back the same tables and the same
+electron records; a file built for other parameters is refused (the
+reference silently reuses a stale '{type}_{shell}.table', Q12)."""
+import os
+
+import numpy as np
+import pytest
+
+import tables_ref as TR
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no GPU")]
+
+C1 = dict(material="Si", E0=10000.0, Emin=1000.0, Wc=100.0, tol_E=1e-2, tol_W=1e-2, s_tilt=70.0)
+C2 = dict(material="Cu", E0=20000.0, Emin=1000.0, Wc=50.0, tol_E=1e-3, tol_W=1e-3, s_tilt=70.0)
+
+
+def _tr(cfg, **kw):
+    from mcd.engine import Transport
+    c = dict(cfg)
+    return Transport(c.pop("material"), c.pop("E0"), c.pop("Emin"), **c, **kw)
+
+
+@pytest.mark.parametrize("cfg", [C1, C2], ids=["Si10", "Cu20"])
+def test_library_tables_equal_python_built_tables(cfg):
+    lib_tr, py_tr = _tr(cfg, grid_search="library"), _tr(cfg)
+    for q in (0, 1):
+        a, b = lib_tr.grids[q], py_tr.grids[q]
+        assert np.array_equal(a.Es, b.Es) and np.array_equal(a.Ws, b.Ws)
+        ca, cb = lib_tr.cdf(q), py_tr.cdf(q)
+        assert np.array_equal(ca, cb)
+        mat = lib_tr.mat
+        ref = TR.cdf_table(a.Es, a.Ws, mat.fermi_e, "Moller" if q == 0 else "Gryzinski", a.n_el, a.Eb)
+        assert np.abs(ca - ref).max() <= 1e-5
+
+
+def test_table_file_round_trip_and_records(tmp_path):
+    import torch
+    from mcd import _abi
+    built = _tr(C2, grid_search="library")
+    path = str(tmp_path / "cu20.mcdt")
+    built.save_tables(path)
+    loaded = _tr(C2, table_file=path)
+    for q in (0, 1):
+        assert np.array_equal(built.grids[q].Es, loaded.grids[q].Es)
+        assert np.array_equal(built.grids[q].Ws, loaded.grids[q].Ws)
+        assert np.array_equal(built.cdf(q), loaded.cdf(q))
+    n = 1 << 14
+    dev = torch.device("cuda", 0)
+    outs = []
+    for tr in (built, loaded):
+        els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+        tr.launch(n, els, cnt, seed=1)
+        torch.cuda.synchronize(dev)
+        outs.append(els.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_table_file_for_other_parameters_is_refused(tmp_path):
+    from mcd import _abi
+    from mcd._lib import MCDError
+    from mcd.material import Material
+    path = str(tmp_path / "si10.mcdt")
+    _tr(C1, grid_search="library").save_tables(path)
+    for change in (dict(Wc=101.0), dict(tol_W=2e-2), dict(Emin=900.0), dict(material=Material("Si", 1.01))):
+        cfg = dict(C1)
+        cfg.update(change)
+        with pytest.raises(MCDError) as e:
+            _tr(cfg, table_file=path)
+        assert e.value.code == _abi.ERR_TABLE and "other parameters" in str(e.value)
+    raw = open(path, "rb").read()
+    bad = str(tmp_path / "short.mcdt")
+    open(bad, "wb").write(raw[:-100])
+    with pytest.raises(MCDError) as e:
+        _tr(C1, table_file=bad)
+    assert e.value.code == _abi.ERR_TABLE
+    open(bad, "wb").write(b"not a table file")
+    with pytest.raises(MCDError) as e:
+        _tr(C1, table_file=bad)
+    assert e.value.code == _abi.ERR_TABLE
+    # tables built from given grids carry no tolerances: nothing to key a file with
+    with pytest.raises(MCDError):
+        _tr(C1).save_tables(str(tmp_path / "x.mcdt"))
+
+
+def test_gen_tables_no_reads_the_keyed_file(tmp_path):
+    """doScatter.py with gen_tables: yes writes the keyed table file; the same
+    input with gen_tables: no reads it back and gives the same store; a run
+    with another Wc and gen_tables: no gets its own file, not the stale one."""
+    import doScatter
+    from test_gpu_dropin import read_store
+    text = ("num_el : 600\nE0 : 10000\nEmin : 1000\nmaterial : Si\ns_tilt : 70\nelastic : Ruth_vanilla\n"
+            "mode : DS\nBethe_model : classical\ndiffMFP : no\ngen_tables : {gen}\ntol_E : 1e-2\ntol_W : 1e-2\n"
+            "maxScatt : off\nWc : {wc}\nseed : 0\nelectron_output : outcome, final_E\nscatter_output : type, E_loss\n")
+    out = tmp_path / "out"
+    f = tmp_path / "in.file"
+    f.write_text(text.format(gen="yes", wc=100))
+    p1 = doScatter.main(["-i", str(f), "-o", str(out), "-c", "256"])
+    files = sorted(os.listdir(out / "tables"))
+    assert len(files) == 1 and files[0].endswith(".mcdt")
+    mtime = os.path.getmtime(out / "tables" / files[0])
+    a_els, a_scats = read_store(p1, "els"), read_store(p1, "scats")
+    f.write_text(text.format(gen="no", wc=100))
+    p2 = doScatter.main(["-i", str(f), "-o", str(out), "-c", "256"])
+    assert os.path.getmtime(out / "tables" / files[0]) == mtime
+    b_els, b_scats = read_store(p2, "els"), read_store(p2, "scats")
+    assert a_els.equals(b_els) and a_scats.equals(b_scats)
+    f.write_text(text.format(gen="no", wc=120))
+    doScatter.main(["-i", str(f), "-o", str(out), "-c", "256"])
+    assert len(os.listdir(out / "tables")) == 2
