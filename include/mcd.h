@@ -49,6 +49,16 @@ enum {
 /* transport modes (input.file `mode`, `Bethe_model`) */
 enum { MCD_MODE_DS = 0, MCD_MODE_CONT = 1 };
 enum { MCD_BETHE_CLASSICAL = 0, MCD_BETHE_JL = 1, MCD_BETHE_EXPLICIT = 2 };
+/* elastic cross section (the input key 'elastic', MC/tools/scattering.py:483-493,
+ * 780-789; crossSections.py:37-147): Rutherford with Joy's screening, with
+ * Z^2 -> Z(Z+1) for inelastic deflections (wDefl), or with the Nigram
+ * screening parameter.  The polar angle is Rutherford_halfPol in every case. */
+enum {
+  MCD_ELASTIC_RUTH = 0,          /* Ruth_vanilla:       ruther_sigma */
+  MCD_ELASTIC_RUTH_WDEFL = 1,    /* Ruth_vanilla_wDefl: ruther_sigma_wDefl */
+  MCD_ELASTIC_NIGRAM = 2,        /* Ruth_nigram:        ruther_N_sigma */
+  MCD_ELASTIC_NIGRAM_WDEFL = 3   /* Ruth_nigram_wDefl:  ruther_N_sigma_wDefl */
+};
 
 /* electron outcomes (MC/tools/electron.py:44-84, singleScatter.py:15-23) */
 enum {
@@ -111,6 +121,7 @@ typedef struct mcd_config {
   double J;          /* mean ionisation energy for the Bethe laws, eV (stoppingPowers.py:134-137) */
   double explicit_lnEstar; /* explicit Bethe: sum_i n_i ln E_i / Z_eff (host-computed, FP64) */
   double explicit_Zeff;    /* explicit Bethe: n_val + sum_i n_i */
+  int32_t elastic;         /* MCD_ELASTIC_* */
 } mcd_config;
 
 /* Inverse-CDF energy-loss table grid (probTables.py:70-535).  Es has nE

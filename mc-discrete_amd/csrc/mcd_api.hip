@@ -170,6 +170,8 @@ static int check_config(mcd_ctx* ctx, const mcd_config* cfg) {
   if (cfg->mode == MCD_MODE_CONT && (cfg->bethe_model < 0 || cfg->bethe_model > 2))
     return set_err(ctx, MCD_ERR_INVALID, "unknown Bethe model %d", cfg->bethe_model);
   if (!(cfg->thickness > 0.)) return set_err(ctx, MCD_ERR_INVALID, "thickness must be > 0");
+  if (cfg->elastic < MCD_ELASTIC_RUTH || cfg->elastic > MCD_ELASTIC_NIGRAM_WDEFL)
+    return set_err(ctx, MCD_ERR_INVALID, "unknown elastic model %d", cfg->elastic);
   if (cfg->mode == MCD_MODE_CONT) {
     // the Bethe laws must remove energy down to Emin, or a trajectory never ends
     const double E = cfg->Emin, J = cfg->J;
@@ -198,7 +200,12 @@ static DevCfg make_devcfg(const mcd_config& cfg, const mcd_tables* t, uint32_t c
   c.Ef = m.fermi_e; c.Epl = m.plasmon_e; c.atnd = m.atnd;
   c.thickness = cfg.thickness;
   c.c_alpha = 3.4 * pow((double)m.Z, 0.666667);       // crossSections.py:27
-  c.ZZ = (double)(m.Z * m.Z);
+  c.elastic = cfg.elastic;
+  c.ZZ = cfg.elastic == MCD_ELASTIC_RUTH_WDEFL ? (double)(m.Z * (m.Z + 1)) : (double)(m.Z * m.Z);
+  // Nigram screening (crossSections.py:113-116, 140-145): beta* = 0.48 * 5.43 Z^(2/3) / E
+  c.nig_zp = 5.43 * pow((double)m.Z, 2. / 3.);
+  c.nig_num = cfg.elastic == MCD_ELASTIC_NIGRAM_WDEFL ? kPiEFour * (double)(m.Z + 1) * (double)m.Z
+                                                      : kPiEFour * (double)(m.Z * m.Z);
   c.n_val = m.n_val;
   c.J = cfg.J; c.bethe_kk = m.bethe_k;
   c.zeff = cfg.explicit_Zeff; c.lnEstar = cfg.explicit_lnEstar; c.E_val = m.E_val;
@@ -213,6 +220,8 @@ static DevCfg make_devcfg(const mcd_config& cfg, const mcd_tables* t, uint32_t c
   c.c_alpha_f = (float)c.c_alpha;
   c.ruth_k_f = (float)(5.21e-21 * c.ZZ * 4. * 3.141592653589793);
   c.mfp_k_f = (float)(1e14 / m.atnd);
+  c.nig_zp_f = (float)c.nig_zp;
+  c.nig_num_f = (float)c.nig_num;
   c.bethe_k1_f = (float)(2. * kPiEFour * m.atnd * (double)m.Z * 1e-14);
   c.bethe_c1_f = (float)(1.166 / cfg.J);
   c.bethe_kJ_f = (float)(m.bethe_k * cfg.J);
@@ -746,7 +755,7 @@ static int default_minw(const mcd_config&) {
 
 static bool same_config(const mcd_config& a, const mcd_config& b) {
   return a.mat.Z == b.mat.Z && a.mat.atnd == b.mat.atnd && a.mat.fermi_e == b.mat.fermi_e && a.E0 == b.E0 &&
-         a.Emin == b.Emin && a.Wc == b.Wc && a.mat.n_shells == b.mat.n_shells;
+         a.Emin == b.Emin && a.Wc == b.Wc && a.mat.n_shells == b.mat.n_shells && a.elastic == b.elastic;
 }
 
 // static_park: nullptr for mcd_forward (the deferred guard parks in the

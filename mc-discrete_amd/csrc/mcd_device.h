@@ -80,6 +80,8 @@ struct DevCfg {
   double E0, Emin, Wc, Ef, Epl, atnd, thickness, c_alpha, ZZ, n_val;
   double J, bethe_kk, zeff, lnEstar, E_val;
   double expl_k;  // explicit Bethe law: 2 pi e^4 n Z_eff 1e-14 (S = expl_k (ln E - lnEstar) / E)
+  int elastic;    // MCD_ELASTIC_*
+  double nig_zp, nig_num;  // Nigram sigma: 5.43 Z^(2/3) and pi e^4 Z^2 (or pi e^4 (Z+1) Z)
   double Eb[MCD_MAX_SHELLS], ns[MCD_MAX_SHELLS];
   double dir0[3];
   // FP32 constants
@@ -88,6 +90,7 @@ struct DevCfg {
   float Epl_hi, Epl_lo;
   float thickness_f, c_alpha_f, ruth_k_f, mfp_k_f;
   float bethe_k1_f, bethe_c1_f, bethe_kJ_f, expl_k_f, expl_inv_Estar_f;
+  float nig_zp_f, nig_num_f;
   float dir0_f[3];
   DevTable tm, tg;
   DevPick pick;
@@ -219,7 +222,12 @@ __device__ __forceinline__ bool f_lt_f2(float v, F2 a) { return v < a.hi || (v =
 // ---------------------------------------------------------------------------
 // FP64 cross sections, mirroring crossSections.py operation order.
 #pragma clang fp contract(off)
-__device__ __forceinline__ double d_sigma_ruth(double E, const DevCfg& c) {  // crossSections.py:37-58
+__device__ __forceinline__ double d_sigma_ruth(double E, const DevCfg& c) {  // crossSections.py:37-147
+  if (c.elastic >= MCD_ELASTIC_NIGRAM) {  // ruther_N_sigma(_wDefl), :91-147
+    const double b = 0.48 * (c.nig_zp / E);
+    return c.nig_num / (4. * b * (1. + b) * (E * E));
+  }
+  // ruther_sigma (:37-58), ruther_sigma_wDefl (:62-88): ZZ = Z^2 or Z (Z + 1)
   double a = c.c_alpha / E;
   double Ek = E * 1e-3;
   double r = (Ek + 511.) / (Ek + 1024.);

@@ -405,10 +405,17 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       const float u0 = u01f(r.x), u1 = u01f(r.y);
       u_az = u01f(r.z);
       // Rutherford-only mean free path (scattering.py:780-797)
+      // Joy's screening: the polar angle (Rutherford_halfPol) in every model
       const float a = c.c_alpha_f * __builtin_amdgcn_rcpf(E.hi);
-      const float Ek = E.hi * 1e-3f;
-      const float rr = (Ek + 511.f) * __builtin_amdgcn_rcpf(Ek + 1024.f);
-      const float sig = c.ruth_k_f * rr * rr * __builtin_amdgcn_rcpf(Ek * Ek * a * (1.f + a));
+      float sig;
+      if (c.elastic >= MCD_ELASTIC_NIGRAM) {  // ruther_N_sigma(_wDefl), crossSections.py:91-147
+        const float b = 0.48f * (c.nig_zp_f * __builtin_amdgcn_rcpf(E.hi));
+        sig = c.nig_num_f * __builtin_amdgcn_rcpf(4.f * b * (1.f + b) * (E.hi * E.hi));
+      } else {
+        const float Ek = E.hi * 1e-3f;
+        const float rr = (Ek + 511.f) * __builtin_amdgcn_rcpf(Ek + 1024.f);
+        sig = c.ruth_k_f * rr * rr * __builtin_amdgcn_rcpf(Ek * Ek * a * (1.f + a));
+      }
       const float mfp = c.mfp_k_f * __builtin_amdgcn_rcpf(sig);
       l = -mfp * ln_f(u0);
       if (!(l < 1e4f)) sflags |= MCD_FLAG_LONG_PATH;

@@ -45,6 +45,7 @@ class Config(C.Structure):
         ("mode", C.c_int32), ("bethe_model", C.c_int32),
         ("max_scatt", C.c_int64),
         ("J", C.c_double), ("explicit_lnEstar", C.c_double), ("explicit_Zeff", C.c_double),
+        ("elastic", C.c_int32),
     ]
 
 
@@ -107,8 +108,43 @@ def row_offsets(rows, first):
     return (rows["electron"] - np.uint32(first & 0xFFFFFFFF)).astype(np.int64)
 
 
+ELASTIC_RUTH, ELASTIC_RUTH_WDEFL, ELASTIC_NIGRAM, ELASTIC_NIGRAM_WDEFL = range(4)
+
+
+def elastic_model(elastic, mode) -> int:
+    """The elastic cross section the reference uses for the input key
+    'elastic' in each mode, with its string tests as written:
+    DS (scattering.py:483-493): == 'Ruth_vanilla', then 'vanilla_wDefl' in,
+    then == 'Ruth_nigram', then 'nigram_wDefl' in;
+    continuous (scattering.py:780-789): 'vanilla' in, then 'nigram' in -- so
+    in continuous mode Ruth_vanilla_wDefl gets ruther_sigma and
+    Ruth_nigram_wDefl gets ruther_N_sigma, their wDefl branches unreachable."""
+    if isinstance(elastic, int):
+        return elastic
+    cont = mode in ("cont", "continuous", MODE_CONT)
+    if "Ruth" in elastic:
+        if cont:
+            if "vanilla" in elastic:
+                return ELASTIC_RUTH
+            if "nigram" in elastic:
+                return ELASTIC_NIGRAM
+        else:
+            if elastic == "Ruth_vanilla":
+                return ELASTIC_RUTH
+            if "vanilla_wDefl" in elastic:
+                return ELASTIC_RUTH_WDEFL
+            if elastic == "Ruth_nigram":
+                return ELASTIC_NIGRAM
+            if "nigram_wDefl" in elastic:
+                return ELASTIC_NIGRAM_WDEFL
+        raise ValueError(f"elastic model {elastic!r}: the reference sets no elastic cross section for it in "
+                         f"{'continuous' if cont else 'DS'} mode")
+    raise NotImplementedError(f"elastic model {elastic!r}: only the Rutherford models are implemented "
+                              "(the Mott tables are not part of the reference)")
+
+
 def make_config(mat, E0, Emin, Wc=50.0, s_tilt=0.0, thickness=float("inf"), mode="DS",
-                bethe_model="classical", max_scatt=0, J=None) -> Config:
+                bethe_model="classical", max_scatt=0, J=None, elastic="Ruth_vanilla") -> Config:
     """Fill an mcd_config from a mcd.material.Material and run parameters."""
     from .material import bethe_J
 
@@ -138,6 +174,7 @@ def make_config(mat, E0, Emin, Wc=50.0, s_tilt=0.0, thickness=float("inf"), mode
             raise ValueError(f"unknown mode {mode!r}")
         mode = modes[mode]
     c.mode = int(mode)
+    c.elastic = elastic_model(elastic, c.mode)
     models = {"classical": BETHE_CLASSICAL, "JL": BETHE_JL, "explicit": BETHE_EXPLICIT}
     if isinstance(bethe_model, str):
         if bethe_model not in models:

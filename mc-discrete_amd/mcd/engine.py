@@ -98,7 +98,8 @@ class Transport:
                  mode: str = "DS", bethe_model: str = "classical", tol_E: float = 1e-3, tol_W: float = 1e-3,
                  thickness: float = float("inf"), max_scatt: int = 0, J: float | None = None,
                  table_precision: int = _abi.PREC_FP64, device: int = 0, grids=None, cdfs=None,
-                 pick_nodes: int = 0, grid_search: str = "host", table_file: str | None = None):
+                 pick_nodes: int = 0, grid_search: str = "host", table_file: str | None = None,
+                 elastic: str | int = "Ruth_vanilla"):
         """DS tables: from `grids` (and `cdfs`) when given; else read from
         `table_file` (mcd_tables_load: refused when the file was built for
         other parameters); else built with the grid search run in Python
@@ -106,7 +107,8 @@ class Transport:
         library (grid_search="library", mcd_tables_build)."""
         self.mat = material if isinstance(material, Material) else Material(material)
         self.device = device
-        self.cfg = _abi.make_config(self.mat, E0, Emin, Wc, s_tilt, thickness, mode, bethe_model, max_scatt, J)
+        self.cfg = _abi.make_config(self.mat, E0, Emin, Wc, s_tilt, thickness, mode, bethe_model, max_scatt, J,
+                                    elastic=elastic)
         self.mode = self.cfg.mode
         self.tol_E, self.tol_W = tol_E, tol_W
         self.grids = grids

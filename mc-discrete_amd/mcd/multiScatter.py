@@ -51,7 +51,7 @@ class DeviceTables(dict):
         p = dict(self.inputPar)
         p.update(inputPar)
         mode = mode or p.get("mode", "DS")
-        key = (mode, p.get("Bethe_model", "classical"), float(p.get("s_tilt", 0.0)),
+        key = (mode, p.get("Bethe_model", "classical"), p.get("elastic", "Ruth_vanilla"), float(p.get("s_tilt", 0.0)),
                float(p.get("thickness", float("inf"))), int(p.get("maxScatt", 0) or 0))
         if key not in self._transports:
             mat = _material(material or self.material, p)
@@ -63,7 +63,8 @@ class DeviceTables(dict):
                 mode="DS" if ds else "cont", bethe_model=p.get("Bethe_model", "classical"),
                 tol_E=p.get("tol_E", 1e-3), tol_W=p.get("tol_W", 1e-3),
                 thickness=p.get("thickness", float("inf")), max_scatt=int(p.get("maxScatt", 0) or 0),
-                grids=grids, table_file=self.table_file if from_file else None)
+                grids=grids, table_file=self.table_file if from_file else None,
+                elastic=p.get("elastic", "Ruth_vanilla"))
         return self._transports[key]
 
 
@@ -76,12 +77,16 @@ def _material(material, inputPar):
 
 
 def _check_elastic(inputPar):
-    el = inputPar.get("elastic", "Ruth_vanilla")
-    if el != "Ruth_vanilla":
-        raise NotImplementedError(f"elastic model {el!r}: only Ruth_vanilla is implemented "
-                                  "(Mott data and the Nigram/wDefl variants are not implemented)")
+    """The Rutherford models (Ruth_vanilla, Ruth_vanilla_wDefl, Ruth_nigram,
+    Ruth_nigram_wDefl) resolve as the reference resolves them in each mode
+    (_abi.elastic_model); Mott needs the Ioffe tables, which the reference
+    does not ship.  diffMFP: the reference's branch reads self.xip_g, whose
+    assignment is commented out (scattering.py:511-513), so it raises."""
+    from . import _abi
+    _abi.elastic_model(inputPar.get("elastic", "Ruth_vanilla"), inputPar.get("mode", "DS"))
     if inputPar.get("diffMFP", False):
-        raise NotImplementedError("diffMFP: the diffraction channel is not implemented")
+        raise NotImplementedError("diffMFP: the reference's diffraction branch reads self.xip_g, which is never set "
+                                  "(scattering.py:511-513); not implemented")
 
 
 def table_file(inputPar, table_dir, material=None):

@@ -82,6 +82,26 @@ double oracle_sigma_ruth(double E, int Z) {
   return 5.21e-21 * ((double)(Z * Z) / (Ek * Ek)) * (r * r) * (4. * OR_PI) / (a * (1. + a));
 }
 
+/* The elastic models of the input key 'elastic' (scattering.py:483-493):
+ * ruther_sigma_wDefl (crossSections.py:62-88) replaces Z^2 by Z (Z + 1);
+ * ruther_N_sigma / ruther_N_sigma_wDefl (:91-147) use the Nigram screening
+ * beta* = 0.48 * 5.43 Z^(2/3) / E (E in eV) in
+ * sigma = pi e^4 Z^2 / (4 beta* (1 + beta*) E^2). */
+double oracle_sigma_ruth_model(double E, int Z, int elastic) {
+  if (elastic == MCD_ELASTIC_NIGRAM || elastic == MCD_ELASTIC_NIGRAM_WDEFL) {
+    double beta_N = 5.43 * pow((double)Z, 2. / 3.) / E;
+    double bs = 0.48 * beta_N;
+    double num = elastic == MCD_ELASTIC_NIGRAM_WDEFL ? OR_PI_EFOUR * (double)(Z + 1) * (double)Z
+                                                     : OR_PI_EFOUR * (double)(Z * Z);
+    return num / (4. * bs * (1. + bs) * (E * E));
+  }
+  double a = or_alpha(E, Z);
+  double Ek = E * 1e-3;
+  double r = (Ek + 511.) / (Ek + 1024.);
+  double zz = elastic == MCD_ELASTIC_RUTH_WDEFL ? (double)(Z * (Z + 1)) : (double)(Z * Z);
+  return 5.21e-21 * (zz / (Ek * Ek)) * (r * r) * (4. * OR_PI) / (a * (1. + a));
+}
+
 /* moller_sigma: crossSections.py:183-208 */
 double oracle_sigma_moller(double E, double Wc, double nfree) {
   if (E <= Wc) return 0.;
@@ -114,7 +134,7 @@ double oracle_sigma_quinn(double E, double Epl, double Ef, double n) {
 int oracle_sigma_set(const mcd_config* c, double E, double* sig, int* label) {
   const mcd_material* m = &c->mat;
   int k = 0;
-  sig[k] = oracle_sigma_ruth(E, m->Z); label[k++] = MCD_CH_RUTH;
+  sig[k] = oracle_sigma_ruth_model(E, m->Z, c->elastic); label[k++] = MCD_CH_RUTH;
   sig[k] = oracle_sigma_moller(E, c->Wc, m->n_val); label[k++] = MCD_CH_MOLLER;
   for (int s = 0; s < m->n_shells; ++s) {
     sig[k] = oracle_sigma_gryz(E, m->shell_Eb[s], m->shell_n[s]);
@@ -385,7 +405,7 @@ static void or_trajectory(const mcd_config* c, const oracle_table* tm, const ora
       hp = OR_PI * u[3];
       or_update_energy(&s, W, c->Emin);
     } else {
-      double sr = oracle_sigma_ruth(E, m->Z);
+      double sr = oracle_sigma_ruth_model(E, m->Z, c->elastic);
       double mfp = 1. / (m->atnd * sr) * 1e14;              /* scattering.py:792 */
       ch = MCD_CH_RUTH;
       l = -mfp * log(u[0]);                                 /* :802-813 */

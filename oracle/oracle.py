@@ -56,6 +56,7 @@ def lib():
         d, i, u32, u64 = C.c_double, C.c_int, C.c_uint32, C.c_uint64
         P = C.POINTER
         L.oracle_sigma_ruth.restype = d; L.oracle_sigma_ruth.argtypes = [d, i]
+        L.oracle_sigma_ruth_model.restype = d; L.oracle_sigma_ruth_model.argtypes = [d, i, i]
         L.oracle_sigma_moller.restype = d; L.oracle_sigma_moller.argtypes = [d, d, d]
         L.oracle_sigma_gryz.restype = d; L.oracle_sigma_gryz.argtypes = [d, d, d]
         L.oracle_sigma_quinn.restype = d; L.oracle_sigma_quinn.argtypes = [d, d, d, d]

@@ -77,7 +77,7 @@ def mfp_at(cfg, E):
         sig, _ = O.sigma_set(cfg, float(E))
         tot = float(np.sum(sig))
     else:
-        tot = O.lib().oracle_sigma_ruth(float(E), int(cfg.mat.Z))
+        tot = O.lib().oracle_sigma_ruth_model(float(E), int(cfg.mat.Z), int(cfg.elastic))
     return 1.0 / (cfg.mat.atnd * tot) * 1e14
 
 

@@ -243,6 +243,44 @@ def test_continuous_mode_matches_oracle(tilt, bethe):
     check_values(r.els, o, same)
 
 
+@pytest.mark.parametrize("elastic", ["Ruth_vanilla_wDefl", "Ruth_nigram", "Ruth_nigram_wDefl"])
+def test_ds_elastic_models_match_oracle(si10, elastic):
+    """DS Si 10 keV with each Rutherford model of the input key 'elastic'
+    (scattering.py:483-493; crossSections.py:62-147): the FP64 kernel equals
+    the oracle trajectory by trajectory, the FP32 kernel to the FP32 bar."""
+    tr0, tm, tg = si10
+    tr = transport(SI10, elastic=elastic, grids=tr0.grids)
+    assert tr.cfg.elastic == _abi.elastic_model(elastic, "DS")
+    n = 2000
+    o, _, ev = O.run(tr.cfg, tm, tg, 0, 0, n, nthreads=16)
+    r64 = tr.run(n, seed=0, precision=_abi.PREC_FP64)
+    assert identical(r64.els, o).all() and r64.events == ev
+    r = tr.run(n, seed=0)
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999, same.mean()
+    check_values(r.els, o, same)
+    # a different elastic cross section gives different trajectories
+    base = tr0.run(n, seed=0)
+    assert not np.array_equal(base.els["n_scatt"], r.els["n_scatt"])
+
+
+@pytest.mark.parametrize("elastic", ["Ruth_nigram", "Ruth_nigram_wDefl", "Ruth_vanilla_wDefl"])
+def test_continuous_elastic_models_match_oracle(elastic):
+    """Continuous mode resolves the names as the reference does
+    (scattering.py:780-789: the wDefl names fall back to the base models)."""
+    from mcd import engine
+    tr = engine.Transport("Au", 30000.0, 1000.0, s_tilt=70.0, mode="cont", bethe_model="JL", elastic=elastic)
+    assert tr.cfg.elastic == (_abi.ELASTIC_NIGRAM if "nigram" in elastic else _abi.ELASTIC_RUTH)
+    n = 2000
+    o, _, ev = O.run(tr.cfg, None, None, 2, 0, n, nthreads=16)
+    r64 = tr.run(n, seed=2, precision=_abi.PREC_FP64)
+    assert identical(r64.els, o).all()
+    r = tr.run(n, seed=2)
+    same = identical(r.els, o)
+    assert same.mean() >= 0.999, same.mean()
+    check_values(r.els, o, same)
+
+
 @pytest.mark.parametrize("material,E0,Emin", [("Au", 30000.0, 1000.0), ("Al", 5000.0, 500.0)])
 def test_ds_other_materials_match_oracle(material, E0, Emin):
     """DS parity at 11 channels (Au: Rutherford, Moller, Quinn and eight

@@ -224,3 +224,76 @@ def test_quinn_mean_free_path_scale():
         assert 1000.0 < lam(100000.0) < 3000.0
         r = lam(20000.0) / lam(10000.0)
         assert 1.7 < r < 2.0
+
+
+# ---------------------------------------------------------------------------
+# Elastic cross-section models (crossSections.py:37-147, scattering.py:483-493,
+# 780-789).  The Nigram model is the total of the screened Rutherford law
+# dsigma/dOmega = (Z e^2 / 4E)^2 / (sin^2(theta/2) + beta)^2 with beta* =
+# 0.48 * 5.43 Z^(2/3) / E: its integral over the sphere is
+# pi Z^2 e^4 / (4 E^2 beta (1 + beta)), checked here by quadrature of the
+# differential law, not by the closed form.  The wDefl models replace Z^2 by
+# Z (Z + 1): their ratio to the base model is (Z + 1) / Z exactly.
+
+def _nigram_quadrature(E, Z, wdefl=False):
+    from scipy.integrate import quad
+    beta = 0.48 * 5.43 * Z ** (2 / 3) / E
+    e4 = PI_EFOUR / math.pi                       # e^4 in cm^2 eV^2
+    zz = Z * (Z + 1) if wdefl else Z * Z
+    pref = zz * e4 / (16.0 * E * E)
+
+    def f(th):
+        s = math.sin(0.5 * th) ** 2
+        return 2 * math.pi * math.sin(th) * pref / (s + beta) ** 2
+    # the law is sharply forward-peaked: split at the screening angle
+    th_s = 2 * math.asin(min(1.0, math.sqrt(beta)))
+    pts = sorted({th_s / 10, th_s, min(math.pi, 10 * th_s)})
+    tot, last = 0.0, 0.0
+    for p in pts + [math.pi]:
+        if p > last:
+            tot += quad(f, last, p, epsabs=0, epsrel=1e-12, limit=200)[0]
+            last = p
+    return tot
+
+
+@pytest.mark.parametrize("Z", [13, 14, 29, 79])
+@pytest.mark.parametrize("E", [500.0, 5000.0, 30000.0])
+def test_nigram_sigma_is_the_screened_rutherford_total(E, Z):
+    from mcd import _abi
+    L = O.lib()
+    got = L.oracle_sigma_ruth_model(E, Z, _abi.ELASTIC_NIGRAM)
+    assert got == pytest.approx(_nigram_quadrature(E, Z), rel=1e-9, abs=0)
+    got_w = L.oracle_sigma_ruth_model(E, Z, _abi.ELASTIC_NIGRAM_WDEFL)
+    assert got_w == pytest.approx(_nigram_quadrature(E, Z, wdefl=True), rel=1e-9, abs=0)
+    # a dropped factor 4, a missing (1 + beta) or an unscreened beta fail the pin
+    beta = 0.48 * 5.43 * Z ** (2 / 3) / E
+    for wrong in (got * 4, got * (1 + beta), got * 0.48):
+        assert wrong != pytest.approx(_nigram_quadrature(E, Z), rel=1e-6, abs=0)
+
+
+@pytest.mark.parametrize("Z", [13, 29, 79])
+@pytest.mark.parametrize("E", [1000.0, 20000.0])
+def test_wdefl_models_scale_by_z_plus_one_over_z(E, Z):
+    from mcd import _abi
+    L = O.lib()
+    for base, wd in ((_abi.ELASTIC_RUTH, _abi.ELASTIC_RUTH_WDEFL), (_abi.ELASTIC_NIGRAM, _abi.ELASTIC_NIGRAM_WDEFL)):
+        r = L.oracle_sigma_ruth_model(E, Z, wd) / L.oracle_sigma_ruth_model(E, Z, base)
+        assert r == pytest.approx((Z + 1) / Z, rel=1e-14, abs=0)
+    assert L.oracle_sigma_ruth_model(E, Z, _abi.ELASTIC_RUTH) == L.oracle_sigma_ruth(E, Z)
+
+
+def test_elastic_model_resolution_follows_the_reference_string_tests():
+    """DS tests the names in the order ==, in, ==, in (scattering.py:483-493);
+    continuous mode tests 'vanilla' in, then 'nigram' in (:780-789), so its
+    wDefl names fall back to the base models."""
+    from mcd import _abi
+    em = _abi.elastic_model
+    assert [em(n, "DS") for n in ("Ruth_vanilla", "Ruth_vanilla_wDefl", "Ruth_nigram", "Ruth_nigram_wDefl")] == \
+        [_abi.ELASTIC_RUTH, _abi.ELASTIC_RUTH_WDEFL, _abi.ELASTIC_NIGRAM, _abi.ELASTIC_NIGRAM_WDEFL]
+    assert [em(n, "cont") for n in ("Ruth_vanilla", "Ruth_vanilla_wDefl", "Ruth_nigram", "Ruth_nigram_wDefl")] == \
+        [_abi.ELASTIC_RUTH, _abi.ELASTIC_RUTH, _abi.ELASTIC_NIGRAM, _abi.ELASTIC_NIGRAM]
+    with pytest.raises(ValueError):
+        em("Ruth_nigram_wDelf", "DS")          # the reference's test input spells it so: no elastic sigma in DS
+    assert em("Ruth_nigram_wDelf", "cont") == _abi.ELASTIC_NIGRAM
+    with pytest.raises(NotImplementedError):
+        em("Mott", "DS")
