@@ -248,15 +248,16 @@ def batched_ratio(reps=3, tune=True):
 
     t_seq = best(seq)
     tuned = (autotune.tune_batch(items, seed=3, reps=1, use_cache=False) if tune else
-             dict(block=128, pick_nodes=128, guard="inline", ms={}))
+             dict(block=128, pick_nodes=128, guard="inline", refill_period=1, ms={}))
     b = engine.Batch(items, pick_nodes=tuned["pick_nodes"])
-    t_bat = best(lambda: b.launch(seed=3, use_graph=True, block=tuned["block"], guard=tuned["guard"]))
+    t_bat = best(lambda: b.launch(seed=3, use_graph=True, block=tuned["block"], guard=tuned["guard"],
+                                  refill_period=tuned.get("refill_period", 1)))
     return {"workload": "64 configurations {Al,Si,Cu,Au} x E0 {5,10,20,30} keV x tilt {0,70} deg x "
                         "{DS, continuous JL}, Emin 0.5 keV, Wc 100 eV, tol 1e-3, "
                         f"{sum(c['n'] for c in cfgs)} electrons (log-uniform 2^14..2^20, seed 3)",
             "sequential_ms": t_seq, "batched_ms": t_bat, "ratio": t_seq / t_bat,
             "tuned": {"block": tuned["block"], "pick_nodes": tuned["pick_nodes"], "guard": tuned["guard"],
-                      "candidates_ms": tuned["ms"]}}
+                      "refill_period": tuned.get("refill_period", 1), "candidates_ms": tuned["ms"]}}
 
 
 def config_sweep(steps, tune=True):

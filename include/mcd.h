@@ -228,10 +228,11 @@ typedef struct mcd_launch {
 #define MCD_LAUNCH_DEFER_GUARD 2
 #define MCD_LAUNCH_SERVICE_GUARD 4
 
-/* Compaction period of mcd_forward: a wavefront refills its idle lanes with
- * new electrons every 2^k steps, k = (flags & MCD_LAUNCH_REFILL_MASK) >>
- * MCD_LAUNCH_REFILL_SHIFT (0 = every step, the default; at most 2^7), and at
- * once when all its lanes are idle.  Records do not depend on it. */
+/* Compaction period of mcd_forward and mcd_batch_run: a wavefront refills its
+ * idle lanes with new electrons every 2^k steps, k = (flags &
+ * MCD_LAUNCH_REFILL_MASK) >> MCD_LAUNCH_REFILL_SHIFT (0 = every step, the
+ * default; at most 2^7), and at once when all its lanes are idle.  Records do
+ * not depend on it. */
 #define MCD_LAUNCH_REFILL_SHIFT 8
 #define MCD_LAUNCH_REFILL_MASK (7 << MCD_LAUNCH_REFILL_SHIFT)
 

@@ -923,6 +923,7 @@ struct mcd_batch {
   uint64_t* graph_counters;
   hipStream_t graph_stream;
   int graph_block, graph_grid, graph_deferred;  // graph_deferred: the guard form the graph was captured with
+  uint32_t graph_refill;                        // the compaction period less one it was captured with
 };
 
 static double work_per_electron(const mcd_config& c) {
@@ -1084,26 +1085,27 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
   return MCD_OK;
 }
 
-// guard placement of a batch run: 0 inline, 1 deferred (phases 1 + 2), 3 served
+// guard placement of a batch run: 0 inline, 1 deferred (phases 1 + 2), 3 served;
+// refill_mask: compaction period less one
 static hipError_t batch_enqueue(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, int form,
-                                int block, hipStream_t s) {
+                                int block, uint32_t refill_mask, hipStream_t s) {
   hipError_t e = launch_fill_u64(b->d_work, 0ull, 4 * b->n_items, s);
   unsigned long long* cnt = (unsigned long long*)counters;
   const bool deferred = form == 1;
   auto run = [&](int phase, int grid) {
-    return launch_batched(b->d_items, b->n_items, b->d_group_start, b->n_groups, b->slice_floats, seed, cnt, phase,
-                          grid, block, s);
+    return launch_batched(b->d_items, b->n_items, b->d_group_start, b->n_groups, b->slice_floats, seed,
+                          refill_mask, cnt, phase, grid, block, s);
   };
   if (form == 3) {
-    const int g3 = ctx->n_cu * batched_blocks_per_cu(3, block, b->slice_floats);
+    const int g3 = ctx->n_cu * batched_blocks_per_cu(3, block, b->slice_floats, refill_mask != 0u);
     if (e == hipSuccess) e = run(3, g3);
   } else if (deferred) {
-    const int g1 = ctx->n_cu * batched_blocks_per_cu(1, block, b->slice_floats);
-    const int g2 = ctx->n_cu * batched_blocks_per_cu(2, block, b->slice_floats);
+    const int g1 = ctx->n_cu * batched_blocks_per_cu(1, block, b->slice_floats, refill_mask != 0u);
+    const int g2 = ctx->n_cu * batched_blocks_per_cu(2, block, b->slice_floats, false);
     if (e == hipSuccess) e = run(1, g1);
     if (e == hipSuccess) e = run(2, g2);
   } else {
-    const int g0 = ctx->n_cu * batched_blocks_per_cu(0, block, b->slice_floats);
+    const int g0 = ctx->n_cu * batched_blocks_per_cu(0, block, b->slice_floats, refill_mask != 0u);
     if (e == hipSuccess) e = run(0, g0);
   }
   if (e == hipSuccess) e = launch_count_outcomes_batched(b->d_items, b->n_items, cnt, s);
@@ -1132,17 +1134,18 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
   } else if (block % 64 || block > 256) {
     return set_err(ctx, MCD_ERR_INVALID, "block %d invalid", block);
   }
+  const uint32_t refill = (1u << ((flags & MCD_LAUNCH_REFILL_MASK) >> MCD_LAUNCH_REFILL_SHIFT)) - 1u;
   hipStream_t s = (hipStream_t)stream;
   if (!use_graph) {
-    HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, form, block, s));
+    HIP_TRY(ctx, batch_enqueue(ctx, b, seed, counters, form, block, refill, s));
     return MCD_OK;
   }
   if (!b->graph || b->graph_seed != seed || b->graph_counters != counters || b->graph_block != block ||
-      b->graph_deferred != form) {
+      b->graph_deferred != form || b->graph_refill != refill) {
     if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
     if (!b->capture_stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
     HIP_TRY(ctx, hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = batch_enqueue(ctx, b, seed, counters, form, block, b->capture_stream);
+    hipError_t e = batch_enqueue(ctx, b, seed, counters, form, block, refill, b->capture_stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
     HIP_TRY(ctx, e);
@@ -1154,6 +1157,7 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
     b->graph_counters = counters;
     b->graph_block = block;
     b->graph_deferred = form;
+    b->graph_refill = refill;
   }
   HIP_TRY(ctx, hipGraphLaunch(b->graph, s));
   return MCD_OK;

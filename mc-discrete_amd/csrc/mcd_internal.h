@@ -50,9 +50,9 @@ struct BatchItemDev {
 constexpr int kMaxXcdGroups = 16;
 
 hipError_t launch_batched(const BatchItemDev* items, int n_items, const int* group_start, int n_groups,
-                          int slice_floats, uint64_t seed, unsigned long long* counters, int phase, int grid,
-                          int block, hipStream_t s);
-int batched_blocks_per_cu(int phase, int block, int slice_floats);
+                          int slice_floats, uint64_t seed, uint32_t refill_mask, unsigned long long* counters,
+                          int phase, int grid, int block, hipStream_t s);
+int batched_blocks_per_cu(int phase, int block, int slice_floats, bool period);
 int batched_cfg_floats();
 
 

@@ -613,13 +613,19 @@ constexpr int kCfgFloats = (int)((sizeof(DevCfg) + 15) / 16 * 4);
 
 int batched_cfg_floats() { return kCfgFloats; }
 
-template <int MODE, int NCH, int PHASE, bool EXPL = false>
+// PERIOD: idle lanes take new electrons every refill_mask + 1 steps (the
+// launch's compaction period), and at once when all are idle; otherwise every
+// step.  A separate instantiation, so the every-step kernels keep their
+// register budget.
+template <int MODE, int NCH, int PHASE, bool PERIOD, bool EXPL = false>
 __device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, const float* pick, uint64_t seed,
-                                         uint32_t& ne, uint32_t& nf, GuardBox* box, uint32_t* pending) {
+                                         uint32_t refill_mask, uint32_t& ne, uint32_t& nf, GuardBox* box,
+                                         uint32_t* pending) {
   RunArgs R = itR;
   R.seed = seed;
+  R.refill_mask = refill_mask;
   // c is the wavefront's LDS copy, so its two table descriptors are in LDS too
-  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0, false, EXPL>(c, pick, &c.tm, R, ne, nf, box, pending);
+  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0, PERIOD, EXPL>(c, pick, &c.tm, R, ne, nf, box, pending);
 }
 
 // The batched kernel's service wavefront: as guard_service, for mailboxes
@@ -688,10 +694,10 @@ static size_t batched_lds_bytes(int phase, int block, int slice_floats) {
 // Register budgets measured on MI355X for the 64-configuration batch: 4
 // resident wavefronts per SIMD for phases 0 and 2, 5 for phase 1 (the FP64-free
 // phase), each faster than the compiler's own allocation despite small spills.
-template <int PHASE>
+template <int PHASE, bool PERIOD>
 __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ? 5 : 4) void k_transport_batched(
     const BatchItemDev* __restrict__ items, int n_items, const int* __restrict__ group_start, int n_groups,
-    int slice_floats, uint64_t seed, unsigned long long* __restrict__ counters) {
+    int slice_floats, uint64_t seed, uint32_t refill_mask, unsigned long long* __restrict__ counters) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -757,20 +763,20 @@ __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ?
     if (mode == MCD_MODE_CONT) {
       if (PHASE != 2) {
         if (c.bethe == MCD_BETHE_EXPLICIT)
-          run_item<MCD_MODE_CONT, 4, PHASE, true>(c, it.R, pick, seed, ne, nf, nullptr, nullptr);
+          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD, true>(c, it.R, pick, seed, refill_mask, ne, nf, nullptr, nullptr);
         else
-          run_item<MCD_MODE_CONT, 4, PHASE>(c, it.R, pick, seed, ne, nf, nullptr, nullptr);
+          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, nullptr, nullptr);
       }
     } else {
       switch (nch) {
-        case 4: run_item<MCD_MODE_DS, 4, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
-        case 5: run_item<MCD_MODE_DS, 5, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
-        case 6: run_item<MCD_MODE_DS, 6, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
-        case 7: run_item<MCD_MODE_DS, 7, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
-        case 8: run_item<MCD_MODE_DS, 8, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
-        case 9: run_item<MCD_MODE_DS, 9, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
-        case 10: run_item<MCD_MODE_DS, 10, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
-        default: run_item<MCD_MODE_DS, 11, PHASE>(c, it.R, pick, seed, ne, nf, box, pending); break;
+        case 4: run_item<MCD_MODE_DS, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        case 5: run_item<MCD_MODE_DS, 5, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        case 6: run_item<MCD_MODE_DS, 6, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        case 7: run_item<MCD_MODE_DS, 7, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        case 8: run_item<MCD_MODE_DS, 8, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        case 9: run_item<MCD_MODE_DS, 9, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        case 10: run_item<MCD_MODE_DS, 10, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        default: run_item<MCD_MODE_DS, 11, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
       }
     }
     const uint64_t cnt[2] = {ne, nf};
@@ -787,37 +793,38 @@ __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ?
   }
 }
 
-template <int PHASE>
+template <int PHASE, bool PERIOD>
 static const void* batched_kernel() {
-  return reinterpret_cast<const void*>(&k_transport_batched<PHASE>);
+  return reinterpret_cast<const void*>(&k_transport_batched<PHASE, PERIOD>);
+}
+
+// The kernel of a phase; the phases that take new electrons (0, 1, 3) have a
+// compaction-period form, the resume phase (2) refills every step.
+static const void* batched_kernel(int phase, bool period) {
+  switch (phase) {
+    case 1: return period ? batched_kernel<1, true>() : batched_kernel<1, false>();
+    case 2: return batched_kernel<2, false>();
+    case 3: return period ? batched_kernel<3, true>() : batched_kernel<3, false>();
+    default: return period ? batched_kernel<0, true>() : batched_kernel<0, false>();
+  }
 }
 
 hipError_t launch_batched(const BatchItemDev* items, int n_items, const int* group_start, int n_groups,
-                          int slice_floats, uint64_t seed, unsigned long long* counters, int phase, int grid,
-                          int block, hipStream_t s) {
+                          int slice_floats, uint64_t seed, uint32_t refill_mask, unsigned long long* counters,
+                          int phase, int grid, int block, hipStream_t s) {
   if (phase == 3 && (block < 128 || block > 1024)) return hipErrorInvalidValue;
   if (n_groups < 1 || n_groups > kMaxXcdGroups) return hipErrorInvalidValue;
   const size_t lds = batched_lds_bytes(phase, block, slice_floats);
-  if (phase == 1)
-    hipLaunchKernelGGL(k_transport_batched<1>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
-                       n_groups, slice_floats, seed, counters);
-  else if (phase == 2)
-    hipLaunchKernelGGL(k_transport_batched<2>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
-                       n_groups, slice_floats, seed, counters);
-  else if (phase == 3)
-    hipLaunchKernelGGL(k_transport_batched<3>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
-                       n_groups, slice_floats, seed, counters);
-  else
-    hipLaunchKernelGGL(k_transport_batched<0>, dim3(grid), dim3(block), lds, s, items, n_items, group_start,
-                       n_groups, slice_floats, seed, counters);
-  return hipGetLastError();
+  const void* k = batched_kernel(phase, refill_mask != 0u);
+  void* args[] = {(void*)&items, (void*)&n_items, (void*)&group_start, (void*)&n_groups, (void*)&slice_floats,
+                  (void*)&seed, (void*)&refill_mask, (void*)&counters};
+  return hipLaunchKernel(k, dim3(grid), dim3(block), args, lds, s);
 }
 
-int batched_blocks_per_cu(int phase, int block, int slice_floats) {
+int batched_blocks_per_cu(int phase, int block, int slice_floats, bool period) {
   int nb = 0;
   const size_t lds = batched_lds_bytes(phase, block, slice_floats);
-  const void* k = phase == 1 ? batched_kernel<1>()
-                             : (phase == 2 ? batched_kernel<2>() : (phase == 3 ? batched_kernel<3>() : batched_kernel<0>()));
+  const void* k = batched_kernel(phase, period);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess || nb < 1) nb = 1;
   return nb;
 }

@@ -13,7 +13,7 @@ lanes idle a little longer).  Records are identical for every shape.
 
 Batched runs (``tune_batch``):
 
-The batched kernel has three shape choices that trade against each other:
+The batched kernel has four shape choices that trade against each other:
 
 * ``pick_nodes`` — the channel-pick table layout: nodes per configuration in
   the wavefront's LDS slice.  More nodes give a narrower FP64 guard band and a
@@ -25,8 +25,8 @@ The batched kernel has three shape choices that trade against each other:
   ``"inline"`` (one phase, more registers, fewer resident wavefronts) or
   ``"service"`` (one phase; wavefront 0 of each workgroup settles the picks of
   the others, which carry no FP64 code).
-
-The batched kernel refills idle lanes every step (ballot + prefix count).
+* ``refill_period`` — the compaction period, as for single configurations
+  (idle lanes found by ballot and numbered by a prefix count).
 
 ``tune_batch`` times every candidate on the actual batch (graph replay, best
 of ``reps``), keeps the fastest and caches the choice by a key of the
@@ -48,6 +48,8 @@ CANDIDATES = (
     dict(block=256, pick_nodes=128, guard="inline"),
     dict(block=256, pick_nodes=0, guard="inline"),
     dict(block=256, pick_nodes=256, guard="inline"),
+    dict(block=256, pick_nodes=128, guard="inline", refill_period=2),
+    dict(block=256, pick_nodes=128, guard="inline", refill_period=4),
     dict(block=128, pick_nodes=128, guard="deferred"),
     dict(block=256, pick_nodes=128, guard="deferred"),
     dict(block=256, pick_nodes=0, guard="deferred"),
@@ -55,7 +57,8 @@ CANDIDATES = (
 
 
 def _name(c):
-    return f"block={c['block']},pick_nodes={c['pick_nodes']},guard={c.get('guard') or 'default'}"
+    return (f"block={c['block']},pick_nodes={c['pick_nodes']},guard={c.get('guard') or 'default'},"
+            f"refill_period={c.get('refill_period', 1)}")
 
 
 TRANSPORT_CANDIDATES = (
@@ -171,14 +174,15 @@ def _time_candidate(items, cand, seed, reps, device):
     from .engine import Batch
 
     b = Batch(items, pick_nodes=cand["pick_nodes"], device=device)
-    b.launch(seed=seed, use_graph=True, block=cand["block"], guard=cand.get("guard"))
+    kw = dict(block=cand["block"], guard=cand.get("guard"), refill_period=cand.get("refill_period", 1))
+    b.launch(seed=seed, use_graph=True, **kw)
     torch.cuda.synchronize(device)
     best = float("inf")
     for _ in range(reps):
         b.counters.zero_()
         torch.cuda.synchronize(device)
         t = time.perf_counter()
-        b.launch(seed=seed, use_graph=True, block=cand["block"], guard=cand.get("guard"))
+        b.launch(seed=seed, use_graph=True, **kw)
         torch.cuda.synchronize(device)
         best = min(best, time.perf_counter() - t)
     del b
@@ -188,7 +192,7 @@ def _time_candidate(items, cand, seed, reps, device):
 def tune_batch(items, seed=3, candidates=CANDIDATES, reps=2, device=0, use_cache=True, cache_path=None):
     """Pick the fastest launch shape for `items` (list of (Transport, n)).
 
-    Returns {"block", "pick_nodes", "guard", "ms": {candidate: ms}, "cached": bool}."""
+    Returns {"block", "pick_nodes", "guard", "refill_period", "ms": {candidate: ms}, "cached": bool}."""
     path = cache_path or _cache_path()
     # the key names the candidate set too, so a cached choice is one of today's
     key = batch_key(items) + hashlib.sha1(",".join(_name(c) for c in candidates).encode()).hexdigest()[:8]
@@ -200,7 +204,8 @@ def tune_batch(items, seed=3, candidates=CANDIDATES, reps=2, device=0, use_cache
     for cand in candidates:
         timings[_name(cand)] = 1e3 * _time_candidate(items, cand, seed, reps, device)
     best = min(candidates, key=lambda c: timings[_name(c)])
-    out = dict(block=best["block"], pick_nodes=best["pick_nodes"], guard=best.get("guard"), ms=timings)
+    out = dict(block=best["block"], pick_nodes=best["pick_nodes"], guard=best.get("guard"),
+               refill_period=best.get("refill_period", 1), ms=timings)
     if use_cache:
         _store(path, key, out)
     return dict(out, cached=False)

@@ -380,10 +380,12 @@ class Batch:
         except Exception:
             pass
 
-    def launch(self, seed=3, use_graph=True, block=0, guard=None, stream=None):
-        """guard: None (library default: inline), "inline", "deferred" or "service"."""
+    def launch(self, seed=3, use_graph=True, block=0, guard=None, stream=None, refill_period=0):
+        """guard: None (library default: inline), "inline", "deferred" or "service".
+        refill_period: the compaction period, idle lanes take new electrons
+        every refill_period steps (0: every step)."""
         ctx = context(self.device)
-        la = _abi.Launch(int(block), 0, _abi.PREC_FP32, guard_flags(guard))
+        la = _abi.Launch(int(block), 0, _abi.PREC_FP32, guard_flags(guard) | refill_flags(refill_period))
         s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
         check(lib().mcd_batch_run(ctx, self._h, int(seed), self.counters.data_ptr(), C.byref(la), int(bool(use_graph)),
                                   s), ctx)

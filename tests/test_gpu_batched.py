@@ -89,15 +89,18 @@ def test_every_batch_configuration_fp32_batched_launch_matches_oracle(small_batc
     assert n_same / n_all >= 0.999
 
 
-@pytest.mark.parametrize("use_graph,guard", [(False, None), (True, None), (True, "deferred"), (False, "service"),
-                                             (True, "service")])
-def test_batched_equals_per_configuration_launches(small_batch, use_graph, guard):
+@pytest.mark.parametrize("use_graph,guard,period", [(False, None, 1), (True, None, 1), (True, "deferred", 1),
+                                                    (False, "service", 1), (True, "service", 1), (True, None, 4),
+                                                    (False, "deferred", 2), (True, "service", 2)])
+def test_batched_equals_per_configuration_launches(small_batch, use_graph, guard, period):
+    """Every guard placement and compaction period (idle lanes refilled every
+    `period` steps) gives the per-configuration records."""
     from mcd import _abi, engine
     cfgs, trs = small_batch
     b = engine.Batch([(tr, c["n"]) for tr, c in zip(trs, cfgs)], pick_nodes=128)
     for rep in range(2):  # graph replays must repeat the eager result
         b.counters.zero_()
-        b.launch(seed=3, use_graph=use_graph, guard=guard)
+        b.launch(seed=3, use_graph=use_graph, guard=guard, refill_period=period)
         res = b.results()
         for i, (tr, c, r) in enumerate(zip(trs, cfgs, res)):
             ref = tr.run(c["n"], seed=3, config_id=i)
@@ -131,20 +134,24 @@ def test_autotuner_picks_and_caches_a_launch_shape(small_batch, tmp_path):
     cfgs, trs = small_batch
     items = [(tr, c["n"]) for tr, c in zip(trs, cfgs)]
     cands = (dict(block=256, pick_nodes=128, guard="deferred"), dict(block=128, pick_nodes=128, guard="deferred"),
-             dict(block=256, pick_nodes=128, guard="inline"))
+             dict(block=256, pick_nodes=128, guard="inline"), dict(block=256, pick_nodes=128, guard="inline",
+                                                                   refill_period=2))
     path = str(tmp_path / "tune.json")
     first = autotune.tune_batch(items, candidates=cands, reps=1, cache_path=path)
     assert not first["cached"]
-    assert (first["block"], first["pick_nodes"], first["guard"]) in {(256, 128, "deferred"), (128, 128, "deferred"),
-                                                                     (256, 128, "inline")}
-    assert set(first["ms"]) == {"block=256,pick_nodes=128,guard=deferred", "block=128,pick_nodes=128,guard=deferred",
-                                "block=256,pick_nodes=128,guard=inline"}
+    assert (first["block"], first["pick_nodes"], first["guard"], first["refill_period"]) in {
+        (256, 128, "deferred", 1), (128, 128, "deferred", 1), (256, 128, "inline", 1), (256, 128, "inline", 2)}
+    assert set(first["ms"]) == {"block=256,pick_nodes=128,guard=deferred,refill_period=1",
+                                "block=128,pick_nodes=128,guard=deferred,refill_period=1",
+                                "block=256,pick_nodes=128,guard=inline,refill_period=1",
+                                "block=256,pick_nodes=128,guard=inline,refill_period=2"}
     again = autotune.tune_batch(items, candidates=cands, reps=1, cache_path=path)
     assert again["cached"] and again["block"] == first["block"] and again["guard"] == first["guard"]
+    assert again["refill_period"] == first["refill_period"]
     other = workloads.batched_configs(scale=1 / 128)
     assert autotune.batch_key(items) != autotune.batch_key([(tr, c["n"]) for tr, c in zip(trs, other)])
     b = engine.Batch(items, pick_nodes=first["pick_nodes"])
-    b.launch(seed=3, use_graph=True, block=first["block"], guard=first["guard"])
+    b.launch(seed=3, use_graph=True, block=first["block"], guard=first["guard"], refill_period=first["refill_period"])
     res = b.results()
     for i, (tr, c, r) in enumerate(zip(trs, cfgs, res)):
         assert r.els.tobytes() == tr.run(c["n"], seed=3, config_id=i).els.tobytes(), i
