@@ -225,7 +225,19 @@ def batched_ratio(reps=3, tune=True):
 
     dev = torch.device("cuda", 0)
     cfgs = workloads.batched_configs()
-    trs = workloads.build_transports(cfgs)
+    # the 64 configurations' tables: one library build per configuration
+    # (mcd_tables_build) against one batched build (mcd_tables_build_batch)
+    t = time.perf_counter()
+    one_by_one = [engine.Transport(c["material"], c["E0"], c["Emin"], Wc=c["Wc"], s_tilt=c["s_tilt"], mode=c["mode"],
+                                   bethe_model=c["bethe_model"], tol_E=c["tol"], tol_W=c["tol"],
+                                   grid_search="library") for c in cfgs]
+    torch.cuda.synchronize(dev)
+    t_one = time.perf_counter() - t
+    del one_by_one
+    t = time.perf_counter()
+    trs = workloads.build_transports(cfgs, batch_tables=True)
+    torch.cuda.synchronize(dev)
+    t_batch_tables = time.perf_counter() - t
     items = [(tr, c["n"]) for tr, c in zip(trs, cfgs)]
     els = [torch.empty(c["n"] * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev) for c in cfgs]
     cnt = torch.zeros(len(cfgs) * _abi.NCOUNTERS, dtype=torch.int64, device=dev)
@@ -256,6 +268,10 @@ def batched_ratio(reps=3, tune=True):
                         "{DS, continuous JL}, Emin 0.5 keV, Wc 100 eV, tol 1e-3, "
                         f"{sum(c['n'] for c in cfgs)} electrons (log-uniform 2^14..2^20, seed 3)",
             "sequential_ms": t_seq, "batched_ms": t_bat, "ratio": t_seq / t_bat,
+            "tables": {"per_configuration_ms": t_one * 1e3, "batched_ms": t_batch_tables * 1e3,
+                       "note": "DS tables of the 64 configurations from tol_E = tol_W = 1e-3: grid search, FP64 "
+                               "CDF rows, row records and channel-pick tables; mcd_tables_build per configuration "
+                               "vs one mcd_tables_build_batch"},
             "tuned": {"block": tuned["block"], "pick_nodes": tuned["pick_nodes"], "guard": tuned["guard"],
                       "refill_period": tuned.get("refill_period", 1), "candidates_ms": tuned["ms"]}}
 

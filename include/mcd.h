@@ -292,6 +292,14 @@ int mcd_grid_search(int kind, double Wmin, double n_el, double Eb, double Ef, do
  * energy-loss tables, as mcd_tables_create with NULL grids. */
 int mcd_tables_build(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, int precision,
                      mcd_tables** out);
+/* mcd_tables_build for n configurations at once (genTables, probTables.py:541-614,
+ * for every configuration of a batch): the grid searches run on up to 16 host
+ * threads, then one device launch fills the CDF rows of every table and one
+ * more their row records.  out[i] receives configuration i's tables, equal bit
+ * for bit to mcd_tables_build's; on an error every out[i] is NULL and the
+ * message names the configuration. */
+int mcd_tables_build_batch(mcd_ctx* ctx, const mcd_config* cfgs, int n, double tol_E, double tol_W, int precision,
+                           mcd_tables** out);
 /* The grid of one table (MCD_TABLE_MOLLER / MCD_TABLE_GRYZ): sizes in nE, nW;
  * Es and Ws (capacity nE / nW floats), when not NULL, receive the grid. */
 int mcd_tables_grid(mcd_ctx* ctx, const mcd_tables* t, int which, int32_t* nE, int32_t* nW, float* Es, float* Ws);

@@ -9,7 +9,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mcd_device.h"
@@ -401,15 +403,11 @@ static int build_pick_table(mcd_ctx* ctx, mcd_tables* t, int n_nodes) {
   return MCD_OK;
 }
 
-extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table_grid* moller,
-                                 const mcd_table_grid* gryz, int precision, mcd_tables** out) {
-  if (!ctx || !out) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_create: NULL argument");
+// New tables for a checked configuration; DS: the grids uploaded and the CDF
+// and row-record buffers allocated, not yet filled.
+static int tables_new(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table_grid* moller, const mcd_table_grid* gryz,
+                      mcd_tables** out) {
   *out = nullptr;
-  int rc = check_config(ctx, cfg);
-  if (rc) return rc;
-  if (precision != MCD_PREC_FP64 && precision != MCD_PREC_FP32)
-    return set_err(ctx, MCD_ERR_INVALID, "unknown table precision %d", precision);
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
   mcd_tables* t = new mcd_tables();
   memset(t, 0, sizeof(*t));
   t->device = ctx->device;
@@ -419,12 +417,41 @@ extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
       delete t;
       return set_err(ctx, MCD_ERR_TABLE, "DS mode needs the Moller and Gryzinski table grids");
     }
+    int rc;
     if ((rc = upload_grid(ctx, moller, t->m)) || (rc = upload_grid(ctx, gryz, t->g))) {
       mcd_tables_destroy(t);
       return rc;
     }
     t->has_w = true;
     t->moller_n_el = moller->n_el;
+  }
+  *out = t;
+  return MCD_OK;
+}
+
+// The channel-pick table of DS tables whose CDFs and row records are built
+// (or queued on the null stream), then the wait for the whole build.
+static int tables_finish(mcd_ctx* ctx, mcd_tables* t) {
+  if (!t->has_w) return MCD_OK;
+  int rc = build_pick_table(ctx, t, pick_nodes_default());
+  if (rc) return rc;
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return set_err(ctx, MCD_ERR_DEVICE, "table build: %s", hipGetErrorString(e));
+  return MCD_OK;
+}
+
+extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_table_grid* moller,
+                                 const mcd_table_grid* gryz, int precision, mcd_tables** out) {
+  if (!ctx || !out) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_create: NULL argument");
+  *out = nullptr;
+  int rc = check_config(ctx, cfg);
+  if (rc) return rc;
+  if (precision != MCD_PREC_FP64 && precision != MCD_PREC_FP32)
+    return set_err(ctx, MCD_ERR_INVALID, "unknown table precision %d", precision);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  mcd_tables* t = nullptr;
+  if ((rc = tables_new(ctx, cfg, moller, gryz, &t))) return rc;
+  if (t->has_w) {
     const mcd_table_grid* gr[2] = {moller, gryz};
     TableDev* td[2] = {&t->m, &t->g};
     for (int q = 0; q < 2; ++q) {
@@ -442,15 +469,10 @@ extern "C" int mcd_tables_create(mcd_ctx* ctx, const mcd_config* cfg, const mcd_
         return set_err(ctx, MCD_ERR_DEVICE, "row-record build: %s", hipGetErrorString(e));
       }
     }
-    if ((rc = build_pick_table(ctx, t, pick_nodes_default()))) {
-      mcd_tables_destroy(t);
-      return rc;
-    }
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-      mcd_tables_destroy(t);
-      return set_err(ctx, MCD_ERR_DEVICE, "table build: %s", hipGetErrorString(e));
-    }
+  }
+  if ((rc = tables_finish(ctx, t))) {
+    mcd_tables_destroy(t);
+    return rc;
   }
   *out = t;
   return MCD_OK;
@@ -509,24 +531,41 @@ extern "C" int mcd_tables_download(mcd_ctx* ctx, const mcd_tables* t, int which,
 // ---------------------------------------------------------------------------
 // Tables from input-file parameters: the tolerance-driven grid search
 // (mcd_grid.hip) and the device CDF fill (probTables.genTables, :541-614).
-static int build_grids(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, std::vector<float>* E,
-                       std::vector<float>* W, mcd_table_grid* g) {
+// The two grids of a DS configuration; no context, so searches of several
+// configurations may run on separate threads.  err receives the reason of a
+// failure.
+static int config_grids(const mcd_config* cfg, double tol_E, double tol_W, std::vector<float>* E,
+                        std::vector<float>* W, mcd_table_grid* g, char* err, size_t err_len) {
   const mcd_material& m = cfg->mat;
-  char err[256];
+  char why[200];
   // Moller: losses over [Wc, W_max(E0)], energies over [Emin, E0] (probTables.py:557-575)
   int rc = search_grid(MCD_TABLE_MOLLER, cfg->Wc, m.n_val, 0.0, m.fermi_e, cfg->Emin, cfg->E0, tol_E, tol_W, E[0],
-                       W[0], err, sizeof err);
-  if (rc) return set_err(ctx, rc, "Moller grid search: %s", err);
+                       W[0], why, sizeof why);
+  if (rc) {
+    snprintf(err, err_len, "Moller grid search: %s", why);
+    return rc;
+  }
   // Gryzinski: the outermost shell, the table every Gryzinski event reads
   // (probTables.py:579, scattering.py:625); energies from max(Emin, E_b)
   const int sh = m.n_shells - 1;
   const double Eb = m.shell_Eb[sh], ns = m.shell_n[sh];
   const double lo = cfg->Emin < Eb ? Eb : cfg->Emin;
-  rc = search_grid(MCD_TABLE_GRYZ, Eb, ns, Eb, m.fermi_e, lo, cfg->E0, tol_E, tol_W, E[1], W[1], err, sizeof err);
-  if (rc) return set_err(ctx, rc, "Gryzinski grid search: %s", err);
+  rc = search_grid(MCD_TABLE_GRYZ, Eb, ns, Eb, m.fermi_e, lo, cfg->E0, tol_E, tol_W, E[1], W[1], why, sizeof why);
+  if (rc) {
+    snprintf(err, err_len, "Gryzinski grid search: %s", why);
+    return rc;
+  }
   g[0] = {(int32_t)E[0].size(), (int32_t)W[0].size(), E[0].data(), W[0].data(), cfg->Wc, m.n_val, 0.0,
           MCD_TABLE_MOLLER};
   g[1] = {(int32_t)E[1].size(), (int32_t)W[1].size(), E[1].data(), W[1].data(), Eb, ns, Eb, MCD_TABLE_GRYZ};
+  return MCD_OK;
+}
+
+static int build_grids(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, std::vector<float>* E,
+                       std::vector<float>* W, mcd_table_grid* g) {
+  char err[256];
+  int rc = config_grids(cfg, tol_E, tol_W, E, W, g, err, sizeof err);
+  if (rc) return set_err(ctx, rc, "%s", err);
   return MCD_OK;
 }
 
@@ -575,6 +614,98 @@ extern "C" int mcd_tables_build(mcd_ctx* ctx, const mcd_config* cfg, double tol_
   (*out)->tol_E = tol_E;
   (*out)->tol_W = tol_W;
   (*out)->precision = precision;
+  return MCD_OK;
+}
+
+// Tables of many configurations: the grid searches on host threads, then the
+// CDF rows of every table in one launch and their row records in another.
+extern "C" int mcd_tables_build_batch(mcd_ctx* ctx, const mcd_config* cfgs, int n, double tol_E, double tol_W,
+                                      int precision, mcd_tables** out) {
+  if (!ctx || !cfgs || !out || n < 0) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_build_batch: NULL argument");
+  for (int i = 0; i < n; ++i) out[i] = nullptr;
+  if (!(tol_E > 0.0 && tol_E < 1.0) || !(tol_W > 0.0 && tol_W < 1.0))
+    return set_err(ctx, MCD_ERR_INVALID, "tolerances tol_E = %g, tol_W = %g outside (0, 1)", tol_E, tol_W);
+  if (precision != MCD_PREC_FP64 && precision != MCD_PREC_FP32)
+    return set_err(ctx, MCD_ERR_INVALID, "unknown table precision %d", precision);
+  for (int i = 0; i < n; ++i) {
+    int rc = check_config(ctx, &cfgs[i]);
+    if (rc) {
+      char msg[256];
+      snprintf(msg, sizeof msg, "%s", ctx->err);
+      return set_err(ctx, rc, "configuration %d: %s", i, msg);
+    }
+  }
+  if (n == 0) return MCD_OK;
+  // grid searches: configurations dealt to threads round-robin
+  std::vector<std::array<std::vector<float>, 2>> E(n), W(n);
+  std::vector<std::array<mcd_table_grid, 2>> g(n);
+  std::vector<int> rcs(n, MCD_OK);
+  std::vector<std::array<char, 256>> errs(n);
+  unsigned hw = std::thread::hardware_concurrency();
+  const int n_threads = (int)std::max(1u, std::min({hw ? hw : 1u, 16u, (unsigned)n}));
+  auto work = [&](int tid) {
+    for (int i = tid; i < n; i += n_threads)
+      if (cfgs[i].mode == MCD_MODE_DS)
+        rcs[i] = config_grids(&cfgs[i], tol_E, tol_W, E[i].data(), W[i].data(), g[i].data(), errs[i].data(),
+                              errs[i].size());
+  };
+  {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < n_threads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (std::thread& th : pool) th.join();
+  }
+  for (int i = 0; i < n; ++i)
+    if (rcs[i]) return set_err(ctx, rcs[i], "configuration %d: %s", i, errs[i].data());
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  auto fail = [&](int rc) {
+    for (int i = 0; i < n; ++i)
+      if (out[i]) { mcd_tables_destroy(out[i]); out[i] = nullptr; }
+    return rc;
+  };
+  std::vector<CdfJob> jobs;
+  long long rows = 0;
+  for (int i = 0; i < n; ++i) {
+    const bool ds = cfgs[i].mode == MCD_MODE_DS;
+    int rc = tables_new(ctx, &cfgs[i], ds ? &g[i][0] : nullptr, ds ? &g[i][1] : nullptr, &out[i]);
+    if (rc) return fail(rc);
+    if (!ds) continue;
+    const TableDev* td[2] = {&out[i]->m, &out[i]->g};
+    for (int q = 0; q < 2; ++q) {
+      CdfJob j;
+      j.Es = td[q]->Es;
+      j.Ws = td[q]->Ws;
+      j.cdf = td[q]->cdf;
+      j.rows = td[q]->rows;
+      j.nE = td[q]->nE;
+      j.nW = td[q]->nW;
+      j.kind = g[i][q].kind;
+      j.Ef = cfgs[i].mat.fermi_e;
+      j.n_el = g[i][q].n_el;
+      j.Eb = g[i][q].Eb;
+      j.row0 = rows;
+      rows += j.nE;
+      jobs.push_back(j);
+    }
+  }
+  if (!jobs.empty()) {
+    CdfJob* d_jobs = nullptr;
+    if (hipMalloc(&d_jobs, jobs.size() * sizeof(CdfJob)) != hipSuccess)
+      return fail(set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for %zu table jobs failed", jobs.size()));
+    hipError_t e = hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(CdfJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_build_tables_batched(d_jobs, (int)jobs.size(), rows, precision, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d_jobs);
+    if (e != hipSuccess) return fail(set_err(ctx, MCD_ERR_DEVICE, "batched table build: %s", hipGetErrorString(e)));
+  }
+  for (int i = 0; i < n; ++i) {
+    int rc = tables_finish(ctx, out[i]);
+    if (rc) return fail(rc);
+    out[i]->keyed = true;
+    out[i]->tol_E = tol_E;
+    out[i]->tol_W = tol_W;
+    out[i]->precision = precision;
+  }
   return MCD_OK;
 }
 

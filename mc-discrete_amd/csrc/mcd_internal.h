@@ -105,6 +105,20 @@ hipError_t launch_ratio_score(const float* c0, const float* cp, const float* cm,
                               float* out, hipStream_t s);
 hipError_t launch_row_score(const float* cdf, const float* Es, int nE, int nW, float* out, hipStream_t s);
 hipError_t launch_build_rows(const float* cdf, const float* Es, int nE, int nW, uint8_t* rows, hipStream_t s);
+// One table of a batched build: its device grids and outputs, and the first
+// of its rows in the batch's row numbering (jobs in order, rows consecutive).
+struct CdfJob {
+  const float* Es;
+  const float* Ws;
+  float* cdf;
+  uint8_t* rows;
+  int nE, nW, kind;
+  double Ef, n_el, Eb;
+  long long row0;
+};
+// CDF rows, then row records, of every job's table.
+hipError_t launch_build_tables_batched(const CdfJob* jobs, int n_jobs, long long total_rows, int precision,
+                                       hipStream_t s);
 hipError_t launch_pick_nodes(const DevCfg& c, int n_nodes, double lnE0, double dln, float* nodes, int stride,
                              hipStream_t s);
 hipError_t launch_pick_check(const DevCfg& c, int n_checks, float* err, int* bad, hipStream_t s);

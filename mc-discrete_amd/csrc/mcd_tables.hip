@@ -43,10 +43,10 @@ __device__ __forceinline__ float f_gryz_dcs(float E, float W, float kn, float Eb
   return (eps < 1.f && epsB < 1.f) ? v : 0.f;
 }
 
-__global__ void k_build_cdf(const float* __restrict__ Es, const float* __restrict__ Ws, int nE, int nW, double Ef,
-                            int kind, double n_el, double Eb, int precision, float* __restrict__ cdf) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nE) return;
+// Energy row i of a table's CDF (computeBlock, probTables.py:415-462).
+__device__ __forceinline__ void cdf_row(const float* __restrict__ Es, const float* __restrict__ Ws, int i, int nW,
+                                        double Ef, int kind, double n_el, double Eb, int precision,
+                                        float* __restrict__ cdf) {
   const int nb = nW - 1;
   float* row = cdf + (size_t)i * nb;
   const float E32 = Es[i];
@@ -86,6 +86,34 @@ __global__ void k_build_cdf(const float* __restrict__ Es, const float* __restric
     for (int j = 0; j < nvalid; ++j) row[j] = row[j] / acc;
   }
   for (int j = nvalid; j < nb; ++j) row[j] = kMasked;
+}
+
+__global__ void k_build_cdf(const float* __restrict__ Es, const float* __restrict__ Ws, int nE, int nW, double Ef,
+                            int kind, double n_el, double Eb, int precision, float* __restrict__ cdf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nE) return;
+  cdf_row(Es, Ws, i, nW, Ef, kind, n_el, Eb, precision, cdf);
+}
+
+// The job holding global row g of a batch: jobs cover consecutive row ranges
+// [row0, row0 + nE) in order.
+__device__ __forceinline__ int job_of_row(const CdfJob* __restrict__ jobs, int n_jobs, long long g) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].row0 <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The CDF rows of many tables in one launch: one thread per energy row of
+// any table, the same arithmetic as k_build_cdf.
+__global__ void k_build_cdf_batched(const CdfJob* __restrict__ jobs, int n_jobs, long long total_rows,
+                                    int precision) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_rows) return;
+  const CdfJob& j = jobs[job_of_row(jobs, n_jobs, g)];
+  cdf_row(j.Es, j.Ws, (int)(g - j.row0), j.nW, j.Ef, j.kind, j.n_el, j.Eb, precision, j.cdf);
 }
 
 hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, double Ef, int kind, double n_el,
@@ -156,9 +184,8 @@ hipError_t launch_row_score(const float* cdf, const float* Es, int nE, int nW, f
 // Row records (layout in mcd_device.h): bracket energies and the quantile
 // guide q -> bisect_left(cdf_row, q / G) (masked entries hold kMasked, above
 // every q / G).  One workgroup per row.
-__global__ void k_build_rows(const float* __restrict__ cdf, const float* __restrict__ Es, int nE, int nW, int narrow,
-                             uint8_t* __restrict__ rows) {
-  const int row = blockIdx.x;
+__device__ __forceinline__ void row_record(const float* __restrict__ cdf, const float* __restrict__ Es, int row,
+                                           int nW, int narrow, uint8_t* __restrict__ rows) {
   const int nb = nW - 1;
   const float* r = cdf + (size_t)row * nb;
   uint8_t* rec = rows + (size_t)row * kRowBytes;
@@ -181,6 +208,31 @@ __global__ void k_build_rows(const float* __restrict__ cdf, const float* __restr
     if (narrow) rec[kRowHeader + q] = (uint8_t)lo;
     else reinterpret_cast<uint16_t*>(rec + kRowHeader)[q] = (uint16_t)lo;
   }
+}
+
+__global__ void k_build_rows(const float* __restrict__ cdf, const float* __restrict__ Es, int nE, int nW, int narrow,
+                             uint8_t* __restrict__ rows) {
+  row_record(cdf, Es, blockIdx.x, nW, narrow, rows);
+}
+
+// Row records of many tables in one launch: one workgroup per energy row of
+// any table.
+__global__ void k_build_rows_batched(const CdfJob* __restrict__ jobs, int n_jobs) {
+  const long long g = blockIdx.x;
+  const CdfJob& j = jobs[job_of_row(jobs, n_jobs, g)];
+  row_record(j.cdf, j.Es, (int)(g - j.row0), j.nW, j.nW - 1 <= 255, j.rows);
+}
+
+hipError_t launch_build_tables_batched(const CdfJob* jobs, int n_jobs, long long total_rows, int precision,
+                                       hipStream_t s) {
+  if (n_jobs < 1 || total_rows < 1 || total_rows > 0x7FFFFFFFLL) return hipErrorInvalidValue;
+  const int block = 64;
+  hipLaunchKernelGGL(k_build_cdf_batched, dim3((unsigned)((total_rows + block - 1) / block)), dim3(block), 0, s, jobs,
+                     n_jobs, total_rows, precision);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_build_rows_batched, dim3((unsigned)total_rows), dim3(128), 0, s, jobs, n_jobs);
+  return hipGetLastError();
 }
 
 hipError_t launch_build_rows(const float* cdf, const float* Es, int nE, int nW, uint8_t* rows, hipStream_t s) {

@@ -61,6 +61,8 @@ def lib():
     L.mcd_grid_search.argtypes = [i32, f64, f64, f64, f64, f64, f64, f64, f64, vp, P(i32), vp, P(i32)]
     L.mcd_tables_build.restype = i32
     L.mcd_tables_build.argtypes = [vp, P(_abi.Config), f64, f64, i32, P(vp)]
+    L.mcd_tables_build_batch.restype = i32
+    L.mcd_tables_build_batch.argtypes = [vp, P(_abi.Config), i32, f64, f64, i32, P(vp)]
     L.mcd_tables_grid.restype = i32
     L.mcd_tables_grid.argtypes = [vp, vp, i32, P(i32), P(i32), vp, vp]
     L.mcd_tables_save.restype = i32

@@ -99,12 +99,14 @@ class Transport:
                  thickness: float = float("inf"), max_scatt: int = 0, J: float | None = None,
                  table_precision: int = _abi.PREC_FP64, device: int = 0, grids=None, cdfs=None,
                  pick_nodes: int = 0, grid_search: str = "host", table_file: str | None = None,
-                 elastic: str | int = "Ruth_vanilla"):
+                 elastic: str | int = "Ruth_vanilla", tables: C.c_void_p | None = None):
         """DS tables: from `grids` (and `cdfs`) when given; else read from
         `table_file` (mcd_tables_load: refused when the file was built for
         other parameters); else built with the grid search run in Python
         (grid_search="host", scipy's quad as the reference integral) or by the
-        library (grid_search="library", mcd_tables_build)."""
+        library (grid_search="library", mcd_tables_build).  `tables`: a handle
+        from mcd_tables_build_batch for this configuration, which the
+        Transport then owns (see build_transports_batch)."""
         self.mat = material if isinstance(material, Material) else Material(material)
         self.device = device
         self.cfg = _abi.make_config(self.mat, E0, Emin, Wc, s_tilt, thickness, mode, bethe_model, max_scatt, J,
@@ -118,7 +120,14 @@ class Transport:
         self._tables = C.c_void_p()
         ctx = context(device)
         self.table_precision = int(table_precision)
-        if self.mode == _abi.MODE_DS and self.grids is None and cdfs is None and (
+        if tables is not None:
+            self._tables = tables
+            if self.mode == _abi.MODE_DS:
+                self.grids = (self._library_grid(_abi.TABLE_MOLLER, moller_grid(self.mat, E0, Emin, Wc)),
+                              self._library_grid(_abi.TABLE_GRYZ, gryz_grid(self.mat, E0, Emin)))
+                if pick_nodes:
+                    check(lib().mcd_tables_set_pick_nodes(ctx, self._tables, int(pick_nodes)), ctx)
+        elif self.mode == _abi.MODE_DS and self.grids is None and cdfs is None and (
                 table_file is not None or grid_search == "library"):
             if table_file is not None:
                 check(lib().mcd_tables_load(ctx, C.byref(self.cfg), float(tol_E), float(tol_W), int(table_precision),
@@ -150,6 +159,17 @@ class Transport:
         else:
             check(lib().mcd_tables_create(ctx, C.byref(self.cfg), None, None, int(table_precision),
                                           C.byref(self._tables)), ctx)
+
+    @classmethod
+    def _config_only(cls, material, E0, Emin, Wc=50.0, s_tilt=0.0, mode="DS", bethe_model="classical",
+                     thickness=float("inf"), max_scatt=0, J=None, elastic="Ruth_vanilla", device=0):
+        """An object holding only the mcd_config of these parameters (no tables)."""
+        self = cls.__new__(cls)
+        self.mat = material if isinstance(material, Material) else Material(material)
+        self.cfg = _abi.make_config(self.mat, E0, Emin, Wc, s_tilt, thickness, mode, bethe_model, max_scatt, J,
+                                    elastic=elastic)
+        self._tables = C.c_void_p()
+        return self
 
     def __del__(self):
         try:
@@ -401,6 +421,33 @@ class Batch:
         return out
 
 
+def build_transports_batch(specs, tol_E=1e-3, tol_W=1e-3, table_precision=_abi.PREC_FP64, device=0, pick_nodes=0):
+    """Transports for many configurations with their tables built together
+    (mcd_tables_build_batch: grid searches on host threads, the CDF rows and
+    row records of every table in one device launch each).  specs: list of
+    dicts of Transport's keyword arguments (material, E0, Emin, Wc, s_tilt,
+    mode, bethe_model, ...); the tables equal those of
+    Transport(..., grid_search="library") bit for bit."""
+    built = [Transport._config_only(device=device, **sp) for sp in specs]
+    cfgs = (_abi.Config * len(built))(*[b.cfg for b in built])
+    handles = (C.c_void_p * len(built))()
+    ctx = context(device)
+    check(lib().mcd_tables_build_batch(ctx, cfgs, len(built), float(tol_E), float(tol_W), int(table_precision),
+                                       handles), ctx)
+    out = []
+    try:
+        for i, sp in enumerate(specs):
+            h = C.c_void_p(handles[i])
+            handles[i] = None  # the Transport owns it from here
+            out.append(Transport(tol_E=tol_E, tol_W=tol_W, table_precision=table_precision, device=device,
+                                 pick_nodes=pick_nodes if sp.get("mode", "DS") == "DS" else 0, tables=h, **sp))
+    finally:
+        for h in handles:  # handles no Transport took (a constructor raised)
+            if h:
+                lib().mcd_tables_destroy(C.c_void_p(h))
+    return out
+
+
 def grids_for(mat: Material, E0, Emin, Wc, tol_E, tol_W):
     """The (Moller, outer-shell Gryzinski) grids of a DS configuration."""
     return (moller_grid(mat, E0, Emin, Wc).build_grids(tol_E, tol_W),
@@ -408,4 +455,5 @@ def grids_for(mat: Material, E0, Emin, Wc, tol_E, tol_W):
 
 
 __all__ = ["Transport", "Batch", "RunResult", "context", "grids_for", "TableGrid", "backscatter_stats",
+           "build_transports_batch",
            "GuardError", "check_guard"]

@@ -27,12 +27,23 @@ def batched_configs(seed=3, scale=1.0):
     return out
 
 
-def build_transports(configs, device=0, pick_nodes=0):
+def build_transports(configs, device=0, pick_nodes=0, batch_tables=False):
     """One Transport per configuration; pick_nodes > 0 sets the node count of
     the DS channel-pick tables (the count a batch's items use, so that
-    per-configuration launches reproduce the batch's records)."""
-    from .engine import Transport
+    per-configuration launches reproduce the batch's records).
+    batch_tables: build every configuration's tables in one library call
+    (mcd_tables_build_batch; the configurations must share one tolerance),
+    instead of one grid search in Python per configuration."""
+    from .engine import Transport, build_transports_batch
 
+    if batch_tables:
+        tols = {c["tol"] for c in configs}
+        if len(tols) != 1:
+            raise ValueError("batch_tables needs one tolerance for every configuration")
+        tol = tols.pop()
+        specs = [dict(material=c["material"], E0=c["E0"], Emin=c["Emin"], Wc=c["Wc"], s_tilt=c["s_tilt"],
+                      mode=c["mode"], bethe_model=c["bethe_model"]) for c in configs]
+        return build_transports_batch(specs, tol_E=tol, tol_W=tol, device=device, pick_nodes=pick_nodes)
     return [Transport(c["material"], c["E0"], c["Emin"], Wc=c["Wc"], s_tilt=c["s_tilt"], mode=c["mode"],
                       bethe_model=c["bethe_model"], tol_E=c["tol"], tol_W=c["tol"], device=device,
                       pick_nodes=pick_nodes if c["mode"] == "DS" else 0)

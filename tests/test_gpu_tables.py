@@ -116,3 +116,61 @@ def test_gen_tables_no_reads_the_keyed_file(tmp_path):
     f.write_text(text.format(gen="no", wc=120))
     doScatter.main(["-i", str(f), "-o", str(out), "-c", "256"])
     assert len(os.listdir(out / "tables")) == 2
+
+
+def test_batched_build_equals_per_configuration_builds():
+    """mcd_tables_build_batch (grid searches on host threads, one device launch
+    for the CDF rows of every table and one for their row records) gives each
+    configuration the tables mcd_tables_build gives it: grids, CDFs and
+    channel-pick tables bit for bit, and the same records; continuous
+    configurations in the batch get their tables too.  The 64-configuration
+    workload's DS tables at tol 1e-3, in FP64 and in the reference's float32
+    arithmetic."""
+    import torch
+    from mcd import _abi, workloads
+    from mcd.engine import Transport, build_transports_batch
+    cfgs = workloads.batched_configs()
+    ds = [c for c in cfgs if c["mode"] == "DS" and c["s_tilt"] == 70.0]
+    specs = [dict(material=c["material"], E0=c["E0"], Emin=c["Emin"], Wc=c["Wc"], s_tilt=c["s_tilt"], mode="DS")
+             for c in ds]
+    specs.append(dict(material="Au", E0=30000.0, Emin=1000.0, s_tilt=0.0, mode="cont", bethe_model="JL"))
+    dev = torch.device("cuda", 0)
+    for prec in (_abi.PREC_FP64, _abi.PREC_FP32):
+        batch = build_transports_batch(specs, tol_E=1e-3, tol_W=1e-3, table_precision=prec)
+        assert len(batch) == len(specs)
+        for sp, b in zip(specs, batch):
+            one = Transport(tol_E=1e-3, tol_W=1e-3, table_precision=prec, grid_search="library", **sp)
+            if sp["mode"] == "DS":
+                for q in (0, 1):
+                    assert np.array_equal(b.grids[q].Es, one.grids[q].Es) and np.array_equal(b.grids[q].Ws,
+                                                                                             one.grids[q].Ws)
+                    assert np.array_equal(b.cdf(q), one.cdf(q)), (sp, q, prec)
+                assert b.pick_info() == one.pick_info()
+            if prec == _abi.PREC_FP64:
+                n = 1 << 12
+                outs = []
+                for tr in (b, one):
+                    els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+                    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+                    tr.launch(n, els, cnt, seed=5)
+                    torch.cuda.synchronize(dev)
+                    outs.append(els.cpu().numpy())
+                assert np.array_equal(outs[0], outs[1]), sp
+        del batch
+
+
+def test_batched_build_names_the_failing_configuration(tmp_path):
+    from mcd import _abi
+    from mcd._lib import MCDError
+    from mcd.engine import build_transports_batch
+    specs = [dict(material="Si", E0=10000.0, Emin=1000.0, Wc=100.0, s_tilt=70.0, mode="DS"),
+             dict(material="Cu", E0=20000.0, Emin=1000.0, Wc=50.0, s_tilt=70.0, mode="DS")]
+    good = build_transports_batch(specs, tol_E=1e-2, tol_W=1e-2)
+    good[1].save_tables(str(tmp_path / "cu.mcdt"))  # keyed tables, as from mcd_tables_build
+    with pytest.raises(MCDError) as e:
+        build_transports_batch(specs, tol_E=0.0, tol_W=1e-2)
+    assert e.value.code == _abi.ERR_INVALID
+    bad = specs + [dict(material="Al", E0=5000.0, Emin=6000.0, Wc=100.0, mode="DS")]  # Emin above E0
+    with pytest.raises(MCDError) as e:
+        build_transports_batch(bad, tol_E=1e-2, tol_W=1e-2)
+    assert "configuration 2" in str(e.value)
