@@ -6,6 +6,7 @@
 // with respect to three parameters per mode (slot -> output column):
 //   DS:         slot 0 = Wc, slot 1 = s_tilt [deg], slot 2 = ln(density)
 //   continuous: slot 0 = J,  slot 1 = s_tilt [deg], slot 2 = ln(density)
+//               (identically zero on a semi-infinite target: not carried)
 //
 // The replay regenerates every draw from the electron's Philox counters
 // (electron, config, step) instead of storing trajectories, and estimates
@@ -176,6 +177,11 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
   const float Jf = (float)c.J, kJ = (float)(c.bethe_kk * c.J);
   const float inv_J = 1.f / Jf;
   const float* gup = hist;  // VJP: [dL/dY, dL/dH_0 ..]
+  // Slots carried.  Continuous mode: free paths scale as 1/n and stopping
+  // powers as n, so on a semi-infinite target (the only kind replayed) no exit
+  // probability, energy or direction depends on the density scale: its slot
+  // (2) is identically zero and is not carried, and its outputs stay 0.
+  constexpr int NS = MODE == MCD_MODE_CONT ? 2 : 3;
 
   // per-lane accumulators: value and 3 slot derivatives
   float accv = 0.f, accg[3] = {0.f, 0.f, 0.f};
@@ -225,9 +231,9 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
 
     const U4 r = philox10((uint32_t)e, c.config_id | (uint32_t)(e >> 32) << 16, step, 0u, k0, k1);
     const float iE = __builtin_amdgcn_rcpf(E);
-    float dlnE[3];
+    float dlnE[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 3; ++i) dlnE[i] = dE[i] * iE;
+    for (int i = 0; i < NS; ++i) dlnE[i] = dE[i] * iE;
     if (MODE == MCD_MODE_DS) dlnE[1] = 0.f;  // DS energies do not depend on the tilt
 
     // node interpolation
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     float dlnl[3];
     dlnl[0] = aE * dlnE[0] + lerp(2);
     dlnl[1] = aE * dlnE[1];
-    dlnl[2] = aE * dlnE[2] + lerp(3);
+    dlnl[2] = NS > 2 ? aE * dlnE[2] + lerp(3) : 0.f;
 
     float u_path, u_ang, u_az;
     int ch = 0;
@@ -268,10 +274,8 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     // ---- exit towards the surface, survival-biased free path ----
     float ddz[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ddz[i] = dd[i][2];
+    for (int i = 0; i < NS; ++i) ddz[i] = dd[i][2];
     if (MODE == MCD_MODE_DS) ddz[1] = kDeg * d[0];  // rigid rotation about y
-    // Accumulate an exit contribution C (value) with pathwise/score part dC,
-    // plus the threshold terms at Emin and at interior bin edges.
     // Accumulate an exit contribution C (value) with its pathwise/score
     // derivative dC into the bin of E, plus the threshold terms: the alive
     // indicator [E > Emin] and the bin indicator, differenced along the energy
@@ -282,10 +286,10 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     auto contribute = [&](float C, const float* dC) {
       const bool base = E > Emin;
       const int b = min((int)(E * A.inv_bin_w), A.n_bins - 1);
-      float tv[3];   // threshold term of the yield
-      int bp[3], bm[3];
+      float tv[3] = {0.f, 0.f, 0.f};   // threshold term of the yield
+      int bp[3] = {-1, -1, -1}, bm[3] = {-1, -1, -1};
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < NS; ++i) {
         const float s = dE[i] * A.delta[i];
         const float Ep = E + s, Em = E - s;
         bp[i] = Ep > Emin ? min((int)(Ep * A.inv_bin_w), A.n_bins - 1) : -1;
@@ -295,11 +299,11 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       if (JAC) {
         if (base) accv += C;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) accg[i] += (base ? dC[i] : 0.f) + tv[i];
+        for (int i = 0; i < NS; ++i) accg[i] += (base ? dC[i] : 0.f) + tv[i];
         float* hb = hist + 4 * b;
         if (base) atomicAdd(&hb[0], C);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < NS; ++i) {
           if (base) atomicAdd(&hb[1 + i], dC[i]);
           if (bp[i] != bm[i]) {
             const float f = C * A.inv_2delta[i];
@@ -311,7 +315,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
         const float g = base ? gup[0] + gup[1 + b] : 0.f;
         accv += C * g;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < NS; ++i) {
           const float gp = bp[i] >= 0 ? gup[0] + gup[1 + bp[i]] : 0.f;
           const float gm = bm[i] >= 0 ? gup[0] + gup[1 + bm[i]] : 0.f;
           accg[i] += dC[i] * g + C * A.inv_2delta[i] * (gp - gm);
@@ -327,17 +331,17 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       const float idz = __builtin_amdgcn_rcpf(d[2]);
       float dx[3], dC[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) dx[i] = rr * dz[i] - x * (ddz[i] * idz + dlnl[i]);
+      for (int i = 0; i < NS; ++i) dx[i] = rr * dz[i] - x * (ddz[i] * idz + dlnl[i]);
       const float C = w * q;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) dC[i] = q * dw[i] + C * (S[i] - dx[i]);
+      for (int i = 0; i < NS; ++i) dC[i] = q * dw[i] + C * (S[i] - dx[i]);
       contribute(C, dC);
       // survival weight and truncated free path
       const float U = q + u_path * (1.f - q);
       l = -lam * ln_f(U);
       const float kq = lam * (1.f - u_path) * q * __builtin_amdgcn_rcpf(U);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < NS; ++i) {
         dl[i] = l * dlnl[i] + kq * dx[i];
         dw[i] = dw[i] * (1.f - q) + w * q * dx[i];
       }
@@ -345,13 +349,13 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     } else {
       l = -lam * ln_f(u_path);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) dl[i] = l * dlnl[i];
+      for (int i = 0; i < NS; ++i) dl[i] = l * dlnl[i];
       if (d[2] < 0.f && z + l * d[2] <= 0.f) {
         // grazing flight (|d_z| < graze) that leaves: counted as it happens,
         // the exit indicator itself is not differentiated
         float dC[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) dC[i] = dw[i] + w * S[i];
+        for (int i = 0; i < NS; ++i) dC[i] = dw[i] + w * S[i];
         contribute(w, dC);
         active = false;
         continue;
@@ -359,7 +363,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
     }
     z += l * d[2];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < NS; ++i) {
       dz[i] += dl[i] * d[2] + l * ddz[i];
       S[i] += sc[i];
     }
@@ -376,7 +380,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       c2 = om * (1.f + a) * inv;
       const float k = -u * om * inv * inv * a;  // ds2/dlnE
 #pragma unroll
-      for (int i = 0; i < 3; ++i) ds2[i] = k * dlnE[i];
+      for (int i = 0; i < NS; ++i) ds2[i] = k * dlnE[i];
     };
     if (MODE == MCD_MODE_DS) {
       if (ch == 0) {
@@ -411,7 +415,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
           s2 = 0.5f * v * __builtin_amdgcn_rcpf(1.f + sq);
           const float k = 0.25f * iE * __builtin_amdgcn_rcpf(sq);
 #pragma unroll
-          for (int i = 0; i < 3; ++i) ds2[i] = k * (dW[i] - v * dE[i]);
+          for (int i = 0; i < NS; ++i) ds2[i] = k * (dW[i] - v * dE[i]);
         }
       }
     } else {
@@ -436,17 +440,17 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
       W = l * Sv;
       const float th[3] = {dSdJ, 0.f, Sv};
 #pragma unroll
-      for (int i = 0; i < 3; ++i) dW[i] = dl[i] * Sv + l * (dSdE * dE[i] + th[i]);
+      for (int i = 0; i < NS; ++i) dW[i] = dl[i] * Sv + l * (dSdE * dE[i] + th[i]);
       ruth(u_ang);
     }
     E -= W;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) dE[i] -= dW[i];
+    for (int i = 0; i < NS; ++i) dE[i] -= dW[i];
     step++;
     // alive at theta or at one side of the threshold stencil theta +- delta
     float Ereach = E;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) Ereach = fmaxf(Ereach, E + fabsf(dE[i] * A.delta[i]));
+    for (int i = 0; i < NS; ++i) Ereach = fmaxf(Ereach, E + fabsf(dE[i] * A.delta[i]));
     if (lost || !(Ereach > Emin) || !(E > 0.f) || !(z > 0.f) || w < kWMin || step >= kBwdStepCap) {
       active = false;
       continue;
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(256) void k_backward(DevCfg c, BwdArgs A) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) { nd[j] *= rd; ny[j] *= ry; }
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < NS; ++i) {
         if (MODE == MCD_MODE_DS && i == 1) continue;
         const float dct = -2.f * ds2[i], dst = kst * ds2[i];
         const float* a = dd[i];
