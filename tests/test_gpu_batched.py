@@ -248,3 +248,28 @@ def test_full_size_batch_properties():
     p = stats.chi2.sf(chi2, len(cfgs))
     print(f"chi2 = {chi2:.1f} over {len(cfgs)} items, p = {p:.3g}")
     assert p > 1e-3
+
+
+@pytest.mark.parametrize("guard", [None, "service"])
+def test_batched_elastic_models_equal_per_configuration_launches(guard):
+    """Items with every Rutherford elastic model (each item's constants,
+    elastic ones included, are copied into its wavefronts' LDS slices), their
+    tables from one mcd_tables_build_batch call, give the per-configuration
+    records."""
+    from mcd import _abi, engine
+    specs = [dict(material="Cu", E0=20000.0, Emin=1000.0, Wc=50.0, s_tilt=70.0, mode="DS", elastic=el)
+             for el in ("Ruth_vanilla", "Ruth_vanilla_wDefl", "Ruth_nigram", "Ruth_nigram_wDefl")]
+    specs += [dict(material="Au", E0=10000.0, Emin=1000.0, s_tilt=0.0, mode="cont", bethe_model="JL", elastic=el)
+              for el in ("Ruth_vanilla", "Ruth_nigram")]
+    trs = engine.build_transports_batch(specs, tol_E=1e-3, tol_W=1e-3, pick_nodes=128)
+    ns = [3000, 2048, 1500, 2500, 1024, 2000]
+    b = engine.Batch(list(zip(trs, ns)), pick_nodes=128)
+    b.launch(seed=7, use_graph=False, guard=guard, block=512 if guard == "service" else 0)
+    res = b.results()
+    yields = []
+    for i, (tr, n, r) in enumerate(zip(trs, ns, res)):
+        ref = tr.run(n, seed=7, config_id=i)
+        assert r.els.tobytes() == ref.els.tobytes(), (i, specs[i]["elastic"])
+        yields.append(float(np.mean(ref.els["outcome"] == _abi.OUT_BKS)))
+    # the models differ: the four DS items do not share one yield
+    assert len({round(y, 6) for y in yields[:4]}) == 4
