@@ -15,9 +15,11 @@
  * MC-discrete repository root).
  *
  * Pinning (see tests/test_oracle_*.py): Philox against the Random123 known-answer
- * vectors; sigma formulas against scipy quad of the differential cross sections
- * and hand-evaluated values; the rotation against an independent closed form and
- * scipy's Rotation; the pick and table lookup against brute force.
+ * vectors; sigma formulas against scipy quad of the differential cross sections,
+ * closed-form limits and hand-evaluated values; the Joy-Luo and explicit Bethe
+ * laws and the Quinn sigma against independent forms and limits, with planted
+ * mistakes rejected; the rotation against an independent closed form and scipy's
+ * Rotation; the pick and table lookup against brute force.
  */
 #include <math.h>
 #include <stdint.h>

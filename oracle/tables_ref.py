@@ -10,6 +10,12 @@ float32 cumulative sum (MaskedArray.cumsum fills with 0) and each row is
 divided by its last cumulative value (:457-460, np.broadcast_to drops the
 mask); masked entries are stored as 0 (mapToMemory, :507).  All arithmetic is
 float32, as the float32 Es/Ws arrays force under numpy 1.x promotion.
+
+Pinning (tests/test_oracle_pins.py, tests/test_oracle_algorithms.py): the
+Moller rows against the analytic Moller CDF; the Gryzinski differential form
+through the limits of its integral at threshold and at high energy (against
+Gryzinski's closed-form total), with planted mistakes rejected, and this
+float32 evaluation against the pinned float64 form to float32 rounding.
 """
 import numpy as np
 

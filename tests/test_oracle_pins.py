@@ -1,5 +1,6 @@
-"""Independent pins of the oracle's Joy-Luo and explicit Bethe stopping powers
-and of its Quinn plasmon cross section.
+"""Independent pins of the oracle's Joy-Luo and explicit Bethe stopping powers,
+of its Quinn plasmon cross section, of the Nigram and wDefl elastic models and
+of the Gryzinski differential cross section of the tables.
 
 The reference's JL and explicit laws (MC/tools/stoppingPowers.py:145-173,
 177-215) are never reached by its own code and none of its tests evaluates
@@ -21,6 +22,11 @@ its own formula:
   k_F (sqrt(1 + E_pl / E_F) - 1)) with scipy's CODATA constants; its
   E >> E_F asymptote with q_min -> E_pl / (2 sqrt(E E_F)) k_F; zero at and
   below the plasmon threshold.
+* Gryzinski's differential form (extFunctions.py:189-219): its integral over
+  [E_b, E] against closed forms at both ends of the energy range, the
+  threshold limit n pi e^4 2^-3/2 (8/9) ln(2.7) (U - 1)^3/2 / E_b^2 and the
+  high-energy limit (Gryzinski's total, crossSections.py:213-242); together
+  they fix every factor of the form.
 
 Every pin is written as a predicate and also applied to deliberately wrong
 variants of the law (a dropped factor, a swapped argument, a wrong sign) to
@@ -32,6 +38,7 @@ import math
 import numpy as np
 import pytest
 from scipy.constants import e as E_CHARGE, hbar, m_e
+from scipy.integrate import quad
 
 import oracle as O
 from mcd import physics
@@ -297,3 +304,107 @@ def test_elastic_model_resolution_follows_the_reference_string_tests():
     assert em("Ruth_nigram_wDelf", "cont") == _abi.ELASTIC_NIGRAM
     with pytest.raises(NotImplementedError):
         em("Mott", "DS")
+
+
+# ---------------------------------------------------------------------------
+# Gryzinski differential cross section (extFunctions.py:189-219)
+#
+# The differential form and Gryzinski's closed-form total (crossSections.py:
+# 213-242, the oracle's oracle_sigma_gryz, pinned by its threshold and
+# high-energy asymptote in test_oracle_rng_physics) describe one cross
+# section; their integral relation is fixed at both ends of the energy range.
+# With U = E / E_b:
+# * U -> 1+: eps, eps_B -> 1, so (1 + eps_B)^-3/2 -> 2^-3/2, the tail factor
+#   (1 - eps)^(eps_B / (eps_B + eps)) -> (1 - eps)^1/2 and the bracket ->
+#   (4/3) ln 2.7; with dW = E d(1 - eps) over [0, U - 1],
+#   integral ~ n pi e^4 2^-3/2 (4/3) ln(2.7) (2/3) (U - 1)^3/2 / E_b^2.
+# * U -> inf: the integral over [E_b, E] tends to the closed-form total (the
+#   leading terms 1/(E E_b) and (2/3) ln sqrt(U) / (E E_b) are the same).
+
+def _gryz_integral(dcs, U, Eb=50.0, n=2.0):
+    import warnings
+
+    from scipy.integrate import IntegrationWarning
+    warnings.simplefilter("ignore", IntegrationWarning)  # quad's roundoff note at 1e-13 near threshold
+    E = U * Eb
+    if U < 1.5:  # near threshold: t = 1 - W/E = s^2 removes the square-root endpoint
+        smax = math.sqrt(1.0 - 1.0 / U)
+        f = lambda s: float(dcs(E, E * (1.0 - s * s), n, Eb)) * E * 2.0 * s  # noqa: E731
+        v, _ = quad(f, 0.0, smax, limit=2000, epsrel=1e-13, epsabs=0.0)
+    else:  # in ln W: the integrand falls like 1/W^2
+        f = lambda t: float(dcs(E, math.exp(t), n, Eb)) * math.exp(t)  # noqa: E731
+        v, _ = quad(f, math.log(Eb), math.log(E), limit=2000, epsrel=1e-13, epsabs=0.0)
+    return v
+
+
+def pin_gryz_dcs(dcs):
+    """True when `dcs(E, W, n_s, E_b)` (float64) has both integral limits."""
+    Eb, n = 50.0, 2.0
+    try:
+        d = 1e-8
+        lead = n * PI_EFOUR * 2.0 ** -1.5 * (4.0 / 3.0) * math.log(2.7) * (2.0 / 3.0) * d ** 1.5 / Eb ** 2
+        if not math.isclose(_gryz_integral(dcs, 1.0 + d, Eb, n), lead, rel_tol=1e-4):
+            return False
+        for U in (1e5, 1e6):
+            tot = O.lib().oracle_sigma_gryz(U * Eb, Eb, n)
+            if not math.isclose(_gryz_integral(dcs, U, Eb, n), tot, rel_tol=1e-4 if U < 1e6 else 1e-5):
+                return False
+    except (ZeroDivisionError, OverflowError, ValueError):
+        return False
+    return True
+
+
+def _gryz_variant(E, W, n, Eb, part):
+    eps, eB = W / E, Eb / E
+    if not (eps < 1.0 and eB < 1.0):
+        return 0.0
+    a = (1.0 + eB) ** -1.5
+    b = (1.0 - eps) ** (eB / (eB + eps))
+    c = (1.0 - eB) + 4.0 * eB * math.log(2.7 + ((1.0 - eps) / eB) ** 0.5) / (3.0 * eps)
+    if part == "no tail factor":
+        b = 1.0
+    elif part == "tail exponent eB/(eB - eps)":
+        b = (1.0 - eps) ** (eB / (eB - eps))
+    elif part == "tail exponent inverted":
+        b = (1.0 - eps) ** ((eB + eps) / eB)
+    elif part == "(1 + eB)^+3/2":
+        a = (1.0 + eB) ** 1.5
+    elif part == "bracket 1 + eB":
+        c = (1.0 + eB) + 4.0 * eB * math.log(2.7 + ((1.0 - eps) / eB) ** 0.5) / (3.0 * eps)
+    elif part == "log coefficient 2/3":
+        c = (1.0 - eB) + 2.0 * eB * math.log(2.7 + ((1.0 - eps) / eB) ** 0.5) / (3.0 * eps)
+    elif part == "ln(e + ...) for ln(2.7 + ...)":
+        c = (1.0 - eB) + 4.0 * eB * math.log(math.e + ((1.0 - eps) / eB) ** 0.5) / (3.0 * eps)
+    elif part == "no square root in the log":
+        c = (1.0 - eB) + 4.0 * eB * math.log(2.7 + (1.0 - eps) / eB) / (3.0 * eps)
+    elif part == "eps^2 under the log term":
+        c = (1.0 - eB) + 4.0 * eB * math.log(2.7 + ((1.0 - eps) / eB) ** 0.5) / (3.0 * eps * eps)
+    elif part == "W^2 for W^3":
+        return n * PI_EFOUR * eps * a * b * c / W ** 2
+    return n * PI_EFOUR * eps * a * b * c / W ** 3
+
+
+def test_gryzinski_dcs_integral_limits():
+    assert pin_gryz_dcs(physics.gryz_dCS)
+    assert pin_gryz_dcs(lambda E, W, n, Eb: _gryz_variant(E, W, n, Eb, None))
+
+
+@pytest.mark.parametrize("part", ["no tail factor", "tail exponent eB/(eB - eps)", "tail exponent inverted",
+                                  "(1 + eB)^+3/2", "bracket 1 + eB", "log coefficient 2/3",
+                                  "ln(e + ...) for ln(2.7 + ...)", "no square root in the log",
+                                  "eps^2 under the log term", "W^2 for W^3"])
+def test_gryzinski_dcs_pins_reject_planted_mistakes(part):
+    assert not pin_gryz_dcs(lambda E, W, n, Eb: _gryz_variant(E, W, n, Eb, part)), part
+
+
+def test_gryzinski_f32_restatement_matches_the_pinned_form():
+    """The oracle's float32 restatement of the table build evaluates the
+    pinned float64 form to float32 rounding (Al 2p, Cu 3p, Au 5p outer shells,
+    losses across the whole W range, energies from above threshold to 30 keV)."""
+    import tables_ref as TR
+    for Eb, ns in ((72.9, 6.0), (74.0, 6.0), (57.0, 6.0)):
+        for E in (1.2 * Eb, 3.0 * Eb, 1000.0, 5000.0, 20000.0, 30000.0):
+            W = np.geomspace(Eb, 0.999 * E, 257).astype(np.float32)
+            got = TR.gryz_dcs_f32(np.float32(E), W, ns, Eb).astype(np.float64)
+            ref = physics.gryz_dCS(float(np.float32(E)), W.astype(np.float64), ns, Eb)
+            assert np.allclose(got, ref, rtol=2e-5, atol=0), (Eb, E)
