@@ -181,6 +181,28 @@ typedef struct mcd_scatter_buf {
   uint64_t max_electron;     /* record only electrons with id < max_electron */
 } mcd_scatter_buf;
 
+/* Fields of mcd_scatter_rec, in record order, for mcd_scatter_pack. */
+enum {
+  MCD_SCAT_ELECTRON = 1u << 0,
+  MCD_SCAT_STEP = 1u << 1,
+  MCD_SCAT_CONFIG = 1u << 2,
+  MCD_SCAT_CHANNEL = 1u << 3,
+  MCD_SCAT_OUTCOME = 1u << 4,
+  MCD_SCAT_E = 1u << 5,
+  MCD_SCAT_E_LOSS = 1u << 6,
+  MCD_SCAT_POL = 1u << 7,
+  MCD_SCAT_AZ = 1u << 8,
+  MCD_SCAT_PATHL = 1u << 9,
+  MCD_SCAT_X = 1u << 10,
+  MCD_SCAT_Y = 1u << 11,
+  MCD_SCAT_Z = 1u << 12,
+  MCD_SCAT_ENERGY = 1u << 13,
+  MCD_SCAT_DX = 1u << 14,
+  MCD_SCAT_DY = 1u << 15,
+  MCD_SCAT_DZ = 1u << 16,
+  MCD_SCAT_FLAGS = 1u << 17
+};
+
 /* Counters written by mcd_forward into caller-owned device memory. */
 enum {
   MCD_CNT_EVENTS = 0,     /* scatter events */
@@ -368,6 +390,18 @@ int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n_items, int
 int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t* counters, const mcd_launch* launch,
                   int use_graph, void* stream);
 int mcd_batch_destroy(mcd_batch* b);
+
+/* The selected fields (a mask of MCD_SCAT_*) of n_rows scatter rows in device
+ * memory, written to device memory as columns: one column per selected field
+ * in record order, n_rows values of the field's type each, every column
+ * starting at a multiple of 256 bytes of out.  *needed receives the bytes the
+ * columns take; out = NULL asks for that size only.  This is the on-device
+ * filter of the scats columns doScatter.py writes (scatter_output,
+ * MC/doScatter.py:112-117; fileTools.py:193-242 keeps only the requested
+ * names): only the requested fields cross to the host.  Asynchronous on
+ * stream. */
+int mcd_scatter_pack(mcd_ctx* ctx, const mcd_scatter_rec* rows, uint64_t n_rows, uint32_t fields, void* out,
+                     size_t out_bytes, size_t* needed, void* stream);
 
 /* Differentiation (no reference counterpart).  Derivatives of
  *   Y   = number of backscattered electrons (outcome 'bks', electron.py:68-70)

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -1291,6 +1292,47 @@ extern "C" int mcd_batch_run(mcd_ctx* ctx, mcd_batch* b, uint64_t seed, uint64_t
     b->graph_refill = refill;
   }
   HIP_TRY(ctx, hipGraphLaunch(b->graph, s));
+  return MCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scatter rows to columns on the device.
+extern "C" int mcd_scatter_pack(mcd_ctx* ctx, const mcd_scatter_rec* rows, uint64_t n_rows, uint32_t fields,
+                                void* out, size_t out_bytes, size_t* needed, void* stream) {
+  if (!ctx || !needed) return set_err(ctx, MCD_ERR_INVALID, "mcd_scatter_pack: NULL argument");
+  if (fields == 0u || (fields >> kScatFields) != 0u)
+    return set_err(ctx, MCD_ERR_INVALID, "field mask 0x%x selects no field or an unknown one", fields);
+  typedef mcd_scatter_rec R;
+  static const int off[kScatFields] = {
+      (int)offsetof(R, electron), (int)offsetof(R, step), (int)offsetof(R, config), (int)offsetof(R, channel),
+      (int)offsetof(R, outcome), (int)offsetof(R, E), (int)offsetof(R, E_loss), (int)offsetof(R, pol),
+      (int)offsetof(R, az), (int)offsetof(R, pathl), (int)offsetof(R, x), (int)offsetof(R, y),
+      (int)offsetof(R, z), (int)offsetof(R, energy), (int)offsetof(R, dx), (int)offsetof(R, dy),
+      (int)offsetof(R, dz), (int)offsetof(R, step_flags)};
+  static const int sz[kScatFields] = {
+      (int)sizeof(R::electron), (int)sizeof(R::step), (int)sizeof(R::config), (int)sizeof(R::channel),
+      (int)sizeof(R::outcome), (int)sizeof(R::E), (int)sizeof(R::E_loss), (int)sizeof(R::pol),
+      (int)sizeof(R::az), (int)sizeof(R::pathl), (int)sizeof(R::x), (int)sizeof(R::y),
+      (int)sizeof(R::z), (int)sizeof(R::energy), (int)sizeof(R::dx), (int)sizeof(R::dy),
+      (int)sizeof(R::dz), (int)sizeof(R::step_flags)};
+  ScatPackPlan plan{};
+  uint64_t at = 0;
+  for (int f = 0; f < kScatFields; ++f) {
+    if (!((fields >> f) & 1u)) continue;
+    plan.src[plan.n] = off[f];
+    plan.size[plan.n] = sz[f];
+    plan.dst[plan.n] = at;
+    at += (n_rows * (uint64_t)sz[f] + 255u) / 256u * 256u;
+    plan.n++;
+  }
+  *needed = (size_t)at;
+  if (!out) return MCD_OK;
+  if (n_rows && !rows) return set_err(ctx, MCD_ERR_INVALID, "mcd_scatter_pack: NULL rows");
+  if (out_bytes < at)
+    return set_err(ctx, MCD_ERR_INVALID, "output holds %zu bytes, the columns need %llu", out_bytes,
+                   (unsigned long long)at);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, launch_pack_scatter(rows, n_rows, plan, out, (hipStream_t)stream));
   return MCD_OK;
 }
 

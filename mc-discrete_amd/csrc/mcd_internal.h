@@ -97,6 +97,17 @@ int svc_blocks_per_cu(const DevCfg& c, int minw, int block);
 bool valid_svc_minw(int w);
 bool valid_minw(int w);
 hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsigned long long* counters, hipStream_t s);
+// Fields of mcd_scatter_rec to pack: source offset and size in the record,
+// destination offset of the field's column in the output.
+constexpr int kScatFields = 18;
+struct ScatPackPlan {
+  int n;
+  int src[kScatFields];
+  int size[kScatFields];
+  uint64_t dst[kScatFields];
+};
+hipError_t launch_pack_scatter(const mcd_scatter_rec* rows, uint64_t n, const ScatPackPlan& plan, void* out,
+                               hipStream_t s);
 
 // tables
 hipError_t launch_build_cdf(const float* Es, const float* Ws, int nE, int nW, double Ef, int kind, double n_el,

@@ -1014,6 +1014,35 @@ hipError_t launch_count_outcomes(const mcd_electron_rec* els, uint64_t n, unsign
   return hipGetLastError();
 }
 
+// Selected fields of scatter rows as packed columns: field f of row i goes to
+// out + plan.dst[f] + i * plan.size[f].  One thread per row, grid-stride; a
+// row is one 64-byte line, so its fields are read from one cache line and
+// each column is written with consecutive addresses across the wavefront.
+__global__ __launch_bounds__(256) void k_pack_scatter(const uint8_t* __restrict__ rows, uint64_t n, ScatPackPlan plan,
+                                                      uint8_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint8_t* r = rows + i * sizeof(mcd_scatter_rec);
+    for (int f = 0; f < plan.n; ++f) {
+      const int sz = plan.size[f];
+      const uint8_t* src = r + plan.src[f];
+      uint8_t* dst = out + plan.dst[f] + i * (uint64_t)sz;
+      if (sz == 4) *reinterpret_cast<uint32_t*>(dst) = *reinterpret_cast<const uint32_t*>(src);
+      else if (sz == 2) *reinterpret_cast<uint16_t*>(dst) = *reinterpret_cast<const uint16_t*>(src);
+      else *dst = *src;
+    }
+  }
+}
+
+hipError_t launch_pack_scatter(const mcd_scatter_rec* rows, uint64_t n, const ScatPackPlan& plan, void* out,
+                               hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_pack_scatter, dim3((unsigned)blocks), dim3(256), 0, s,
+                     reinterpret_cast<const uint8_t*>(rows), n, plan, reinterpret_cast<uint8_t*>(out));
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers
 

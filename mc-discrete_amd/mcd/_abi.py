@@ -98,6 +98,21 @@ SCATTER_DTYPE = np.dtype([
     ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("step_flags", "<u4"),
 ])
 assert SCATTER_DTYPE.itemsize == 64
+# mcd_scatter_pack field bits (MCD_SCAT_*): the record's fields in order
+SCAT_FIELD_BITS = {name: 1 << i for i, name in enumerate(SCATTER_DTYPE.names)}
+# scatter_output names (fileTools.thingsToSave) -> the row fields they read
+SCAT_OUTPUT_FIELDS = {"type": ("channel",), "E": ("E",), "E_loss": ("E_loss",), "pol_angle": ("pol",),
+                      "az_angle": ("az",), "pathl": ("pathl",), "energy": ("energy",),
+                      "position": ("x", "y", "z"), "direction": ("dx", "dy", "dz")}
+
+
+def scat_fields_for(names):
+    """Row fields a scatter_output list needs, plus electron and step (the
+    rows are ordered by them on the host)."""
+    out = {"electron", "step"}
+    for n in names:
+        out.update(SCAT_OUTPUT_FIELDS.get(n, ()))
+    return [f for f in SCATTER_DTYPE.names if f in out]
 
 
 def row_offsets(rows, first):

@@ -246,8 +246,12 @@ class Transport:
         return res
 
     def run(self, n, seed=1, first=0, config_id=0, scats=False, scat_max_electron=None, scat_capacity=None,
-            precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None, refill_period=0) -> RunResult:
-        """Run electrons [first, first+n) and return host results."""
+            precision=_abi.PREC_FP32, block=0, waves_per_simd=0, guard=None, refill_period=0,
+            scat_fields=None) -> RunResult:
+        """Run electrons [first, first+n) and return host results.  scat_fields:
+        the scatter-row fields to bring to the host (mcd_scatter_pack packs them
+        into columns on the device); None brings whole rows.  Either way the
+        rows come back sorted by (electron, step)."""
         dev = torch.device("cuda", self.device)
         els = torch.empty(max(n, 1) * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
@@ -273,10 +277,38 @@ class Transport:
             used = int(counters[_abi.CNT_SCAT_ROWS])
             if used > scat_capacity:
                 raise ScatterOverflow(f"scatter-row buffer overflow: {used} rows requested, capacity {scat_capacity}")
-            srows = rows.cpu().numpy()[: used * _abi.SCATTER_DTYPE.itemsize].view(_abi.SCATTER_DTYPE)
+            if scat_fields is None:
+                srows = rows[: used * _abi.SCATTER_DTYPE.itemsize].cpu().numpy().view(_abi.SCATTER_DTYPE)
+            else:
+                srows = self._packed_rows(rows, used, scat_fields)
             srows = srows[np.lexsort((srows["step"], _abi.row_offsets(srows, first)))]
         return RunResult(els=e, counters=counters, scats=srows)
 
+
+    def _packed_rows(self, rows, used, fields):
+        """The given fields of the first `used` scatter rows (device), packed
+        into columns on the device and copied as such: a structured array of
+        those fields (electron and step always among them)."""
+        names = [f for f in _abi.SCATTER_DTYPE.names if f in set(fields) | {"electron", "step"}]
+        mask = 0
+        for f in names:
+            mask |= _abi.SCAT_FIELD_BITS[f]
+        ctx = context(self.device)
+        need = C.c_size_t(0)
+        check(lib().mcd_scatter_pack(ctx, rows.data_ptr(), used, mask, None, 0, C.byref(need), None), ctx)
+        out = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=rows.device)
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().mcd_scatter_pack(ctx, rows.data_ptr(), used, mask, out.data_ptr(), out.numel(), C.byref(need), s),
+              ctx)
+        host = out.cpu().numpy()
+        dt = np.dtype([(f, _abi.SCATTER_DTYPE.fields[f][0]) for f in names])
+        res = np.empty(used, dtype=dt)
+        at = 0
+        for f in names:
+            t = _abi.SCATTER_DTYPE.fields[f][0]
+            res[f] = host[at: at + used * t.itemsize].view(t)
+            at += (used * t.itemsize + 255) // 256 * 256
+        return res
 
     # -- differentiation ------------------------------------------------------
     def run_chunks(self, n, chunk, seed=1, first=0, streams=2, guard=None):

@@ -24,6 +24,7 @@ import os
 
 import numpy as np
 
+from . import _abi
 from .engine import Transport
 from .fileTools import els_columns, per_electron_dicts, scats_columns, zipDict
 from .material import Material
@@ -82,7 +83,6 @@ def _check_elastic(inputPar):
     (_abi.elastic_model); Mott needs the Ioffe tables, which the reference
     does not ship.  diffMFP: the reference's branch reads self.xip_g, whose
     assignment is commented out (scattering.py:511-513), so it raises."""
-    from . import _abi
     _abi.elastic_model(inputPar.get("elastic", "Ruth_vanilla"), inputPar.get("mode", "DS"))
     if inputPar.get("diffMFP", False):
         raise NotImplementedError("diffMFP: the reference's diffraction branch reads self.xip_g, which is never set "
@@ -148,7 +148,8 @@ def _run(inputPar, numTraj, material, tables, thingsToSave, mode):
     first = tables.claim(numTraj)
     want_scats = bool(thingsToSave.get("scat_output"))
     seed = int(inputPar.get("seed", 0))
-    r = run_rows(tr, int(numTraj), seed, first) if want_scats else tr.run(int(numTraj), seed=seed, first=first)
+    r = (run_rows(tr, int(numTraj), seed, first, scat_fields=_abi.scat_fields_for(thingsToSave["scat_output"]))
+         if want_scats else tr.run(int(numTraj), seed=seed, first=first))
     return per_electron_dicts(r.els, r.scats, tr.mat, thingsToSave, first)
 
 
@@ -193,7 +194,7 @@ def scatterMultiEl_cont(inputPar, thingsToSave, output, num, count=0):
 SCAT_ROWS_DEFAULT = 1 << 26
 
 
-def run_rows(tr, n, seed, first, scat_max_electron=None, scat_capacity=None):
+def run_rows(tr, n, seed, first, scat_max_electron=None, scat_capacity=None, scat_fields=None):
     """Electrons [first, first+n) with their scatter rows, in as many device
     runs as the row buffer needs: when a range's rows overflow the buffer it is
     run again as two halves (records and rows are keyed by electron id and the
@@ -204,13 +205,13 @@ def run_rows(tr, n, seed, first, scat_max_electron=None, scat_capacity=None):
     cap = int(scat_capacity) if scat_capacity else min(SCAT_ROWS_DEFAULT, max(1024, 4096 * int(n)))
     try:
         return tr.run(int(n), seed=seed, first=first, scats=True, scat_max_electron=scat_max_electron,
-                      scat_capacity=cap)
+                      scat_capacity=cap, scat_fields=scat_fields)
     except ScatterOverflow:
         if n <= 1:
             raise
     h = int(n) // 2
-    a = run_rows(tr, h, seed, first, scat_max_electron, cap)
-    b = run_rows(tr, int(n) - h, seed, first + h, scat_max_electron, cap)
+    a = run_rows(tr, h, seed, first, scat_max_electron, cap, scat_fields)
+    b = run_rows(tr, int(n) - h, seed, first + h, scat_max_electron, cap, scat_fields)
     return RunResult(els=np.concatenate([a.els, b.els]), counters=a.counters + b.counters,
                      scats=np.concatenate([a.scats, b.scats]))
 
@@ -241,7 +242,7 @@ def run_columns(inputPar, n, tables=None, first=None, scat_max_electron=None, sc
     el_want = list(inputPar.get("electron_output", ("outcome", "final_E", "final_dir", "last_pos")))
     seed = int(inputPar.get("seed", 0))
     if sc_want:
-        r = run_rows(tr, int(n), seed, first, scat_max_electron, scat_capacity)
+        r = run_rows(tr, int(n), seed, first, scat_max_electron, scat_capacity, _abi.scat_fields_for(sc_want))
     else:
         r = tr.run(int(n), seed=seed, first=first)
     els = els_columns(r.els, el_want)

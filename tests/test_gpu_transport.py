@@ -493,6 +493,48 @@ def test_scatter_rows_split_into_ranges_equal_one_run(si10):
     assert (parts.counters[:_abi.CNT_SCAT_ROWS] == whole.counters[:_abi.CNT_SCAT_ROWS]).all()
 
 
+def test_scatter_rows_packed_on_the_device_equal_whole_rows(si10):
+    """mcd_scatter_pack: the requested fields of the scatter rows, packed into
+    columns on the device, equal those fields of the whole rows bit for bit
+    (after the same (electron, step) ordering), for the field sets the
+    scatter_output names need, every field, and ranges split by run_rows;
+    an empty mask, an unknown field and a short buffer are refused."""
+    import ctypes as C
+
+    import torch
+    from mcd import engine
+    from mcd._lib import MCDError, check, lib
+    from mcd.multiScatter import run_rows
+    tr = si10[0]
+    n = 300
+    whole = tr.run(n, seed=2, first=11, scats=True)
+    sets = [_abi.scat_fields_for(["type", "E_loss"]), _abi.scat_fields_for(["position", "energy", "direction"]),
+            list(_abi.SCATTER_DTYPE.names), ["electron", "step"]]
+    for fields in sets:
+        part = tr.run(n, seed=2, first=11, scats=True, scat_fields=fields)
+        assert part.els.tobytes() == whole.els.tobytes()
+        assert set(part.scats.dtype.names) == set(fields) | {"electron", "step"}
+        for f in part.scats.dtype.names:
+            assert part.scats[f].tobytes() == whole.scats[f].tobytes(), (fields, f)
+    split = run_rows(tr, n, 2, 11, scat_capacity=len(whole.scats) // 5, scat_fields=["channel", "E"])
+    for f in split.scats.dtype.names:
+        assert split.scats[f].tobytes() == whole.scats[f].tobytes(), f
+    ctx = engine.context(0)
+    rows = torch.empty(64 * 8, dtype=torch.uint8, device="cuda")
+    need = C.c_size_t(0)
+    for mask in (0, 1 << 18):
+        with pytest.raises(MCDError) as e:
+            check(lib().mcd_scatter_pack(ctx, rows.data_ptr(), 8, mask, None, 0, C.byref(need), None), ctx)
+        assert e.value.code == _abi.ERR_INVALID
+    check(lib().mcd_scatter_pack(ctx, rows.data_ptr(), 8, 0b11, None, 0, C.byref(need), None), ctx)
+    assert need.value == 2 * 256
+    out = torch.empty(need.value - 1, dtype=torch.uint8, device="cuda")
+    with pytest.raises(MCDError) as e:
+        check(lib().mcd_scatter_pack(ctx, rows.data_ptr(), 8, 0b11, out.data_ptr(), out.numel(), C.byref(need), None),
+              ctx)
+    assert e.value.code == _abi.ERR_INVALID
+
+
 def test_reference_statistical_checks_on_gpu_output(si10):
     """The reference's chi-square channel test and Watson U^2 polar-angle test
     (tests/testThings.py:321-434) applied to GPU scatter rows."""
