@@ -15,10 +15,13 @@ format specification (version 3.0 structures):
   one-dimensional compound dataset (``index`` int64, then one field per
   column, as ``data_columns=True`` gives), stored contiguously;
 * the attributes PyTables and pandas put on those objects (CLASS, VERSION,
-  TITLE, FIELD_<i>_NAME, NROWS; pandas_type 'frame_table', table_type
+  TITLE, FIELD_<i>_NAME, NROWS, index_kind and, per data column, <c>_kind,
+  <c>_meta and <c>_dtype; pandas_type 'frame_table', table_type
   'appendable_frame', index_cols, values_cols, non_index_axes, data_columns,
-  nan_rep, encoding, levels).  Object-valued pandas attributes are stored as
-  protocol-0 pickles in string attributes.
+  nan_rep, encoding, levels, info: the attributes pandas' HDFStore reads back
+  for a frame table, pandas/io/pytables.py Table.get_attrs / indexables).
+  Object-valued pandas attributes are stored as protocol-0 pickles in string
+  attributes.
 
 The HDF5 structure is checked against the HDF5 library itself where one is
 installed (the test suite reads these files through libhdf5); that
@@ -265,6 +268,13 @@ class H5Store:
                     tattrs[f"FIELD_{i}_NAME"] = name
                 tattrs["NROWS"] = int(f["nrows"])
                 tattrs["index_kind"] = "integer"
+                # per data column, as pandas' DataCol.set_attr writes them: the
+                # block's column list, the numpy dtype name (bytes<8n> for an
+                # S<n> string column) and the metadata kind (none)
+                for c in cols:
+                    tattrs[f"{c}_kind"] = [c]
+                    tattrs[f"{c}_meta"] = None
+                    tattrs[f"{c}_dtype"] = dt.fields[c][0].name
                 t_addr = out.tell()
                 out.write(_dataset_ohdr(dt, f["nrows"], data_addr[key][0], data_addr[key][1], tattrs))
                 gattrs = {"CLASS": "GROUP", "TITLE": "", "VERSION": "1.0",
@@ -407,7 +417,9 @@ class _Reader:
             val = val.reshape(dims) if dims else val[0]
             if isinstance(val, bytes):
                 s = val.rstrip(b"\0")
-                if s.startswith(b"(") and s.endswith(b"."):
+                if s == b"N.":  # a protocol-0 pickle of None
+                    val = None
+                elif s.startswith(b"(") and s.endswith(b"."):
                     try:
                         val = _plain_loads(s)
                     except Exception:

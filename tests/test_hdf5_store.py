@@ -64,6 +64,15 @@ def test_round_trip(tmp_path):
     assert gattrs["pandas_type"] == "frame_table" and gattrs["table_type"] == "appendable_frame"
     assert gattrs["index_cols"] == [(0, "index")] and gattrs["data_columns"] == list(want)
     assert arr.dtype["outcome"].itemsize == 6
+    # the per-column attributes pandas reads back for a frame table
+    # (pandas/io/pytables.py Table.indexables: <c>_kind, <c>_dtype, <c>_meta)
+    for c in want:
+        assert tattrs[f"{c}_kind"] == [c] and tattrs[f"{c}_meta"] is None
+        assert tattrs[f"{c}_dtype"] == arr.dtype[c].name
+    assert tattrs["outcome_dtype"] == "bytes48" and tattrs["final_E_dtype"] == "float32"
+    import pandas.io.pytables as pt
+    for c in want:  # the kind pandas derives from the stored dtype name
+        pt._dtype_to_kind(tattrs[f"{c}_dtype"])
     assert not glob.glob(path + ".*.part")
 
 
