@@ -1,5 +1,5 @@
 """Parity across the parameter space the input file spans, beyond the named
-workloads: 16 configurations drawn once from a fixed seed (material, mode,
+workloads: 32 configurations drawn once from a fixed seed (material, mode,
 E0 from just above Emin to 25 keV, tilt up to 85 deg, Wc from 10 to 200 eV,
 tolerances 1e-2 and 1e-3, thin films, maxScatt, every Bethe law and elastic
 model a mode reaches).  A DS draw for which the reference's tolerance search
@@ -35,7 +35,7 @@ def _grids_found(c):
     return True
 
 
-def _draw(n=16, seed=20261203):
+def _draw(n=32, seed=20261203):
     rng = np.random.default_rng(seed)
     out = []
     while len(out) < n:
