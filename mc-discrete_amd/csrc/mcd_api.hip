@@ -298,6 +298,14 @@ static int upload_grid(mcd_ctx* ctx, const mcd_table_grid* gr, TableDev& td) {
   return MCD_OK;
 }
 
+// Served-guard wait bound of a launch: kGuardSpinCap, or MCD_GUARD_SPIN_CAP
+// when set to a positive integer (for tests of the timeout path).
+static uint32_t guard_spin_cap() {
+  const char* s = getenv("MCD_GUARD_SPIN_CAP");
+  const long v = s ? atol(s) : 0;
+  return v > 0 && v <= (long)kGuardSpinCap ? (uint32_t)v : kGuardSpinCap;
+}
+
 static int pick_nodes_default() {
   const char* s = getenv("MCD_PICK_NODES");
   int n = s ? atoi(s) : 512;
@@ -936,6 +944,7 @@ static int forward_impl(mcd_ctx* ctx, const mcd_config* cfg, const mcd_tables* t
   }
   const int flags = launch ? launch->flags : 0;
   R.refill_mask = (1u << ((flags & MCD_LAUNCH_REFILL_MASK) >> MCD_LAUNCH_REFILL_SHIFT)) - 1u;
+  R.guard_spins = guard_spin_cap();
   const int guard_flags = flags & (MCD_LAUNCH_INLINE_GUARD | MCD_LAUNCH_DEFER_GUARD | MCD_LAUNCH_SERVICE_GUARD);
   if (guard_flags & (guard_flags - 1))
     return set_err(ctx, MCD_ERR_INVALID, "at most one of MCD_LAUNCH_INLINE_GUARD, _DEFER_GUARD, _SERVICE_GUARD");
@@ -1147,6 +1156,7 @@ extern "C" int mcd_batch_create(mcd_ctx* ctx, const mcd_batch_item* items, int n
     h[i].R.first = 0;
     h[i].R.n = it.n;
     h[i].R.els = it.els;
+    h[i].R.guard_spins = guard_spin_cap();
     work[i] = (double)it.n * work_per_electron(*it.cfg);
     h[i].cost = (float)work_per_electron(*it.cfg);
   }

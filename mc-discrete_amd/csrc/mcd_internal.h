@@ -32,7 +32,15 @@ struct RunArgs {
   unsigned long long* park_count;    // electrons parked by the first kernel
   unsigned long long* resume_queue;  // work counter of the resume kernel
   uint32_t refill_mask;              // idle lanes are refilled every (refill_mask + 1) steps
+  uint32_t guard_spins;              // served guard: polls a transport wavefront waits for a reply
 };
+
+// Default of RunArgs::guard_spins: s_sleep(2) polls of 128 clocks, ~0.2 s in
+// all; past it the step takes the Rutherford channel and is flagged
+// MCD_FLAG_GUARD_TIMEOUT (counted as an error).  The environment variable
+// MCD_GUARD_SPIN_CAP (a positive integer) replaces it, so that a test can
+// drive this error path from the host with every wait still bounded.
+constexpr uint32_t kGuardSpinCap = 1u << 22;
 
 // One configuration of a batched launch (device-resident).  The device array
 // holds the items grouped by XCD (group g = items [group_start[g],

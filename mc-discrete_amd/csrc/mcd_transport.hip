@@ -131,10 +131,9 @@ struct GuardBox {
   GuardSlot lane[64];
 };
 constexpr uint32_t kBoxPosted = 1u, kBoxReplied = 2u;
-// A transport wavefront waits at most this many s_sleep(2) polls (128 clocks
-// each, ~0.2 s in all) for a reply; past it the step takes the Rutherford
+// A transport wavefront waits at most R.guard_spins s_sleep(2) polls for a
+// reply (kGuardSpinCap by default); past it the step takes the Rutherford
 // channel and is flagged MCD_FLAG_GUARD_TIMEOUT (counted as an error).
-constexpr uint32_t kGuardSpinCap = 1u << 22;
 
 __device__ __forceinline__ uint32_t box_load(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -169,7 +168,7 @@ __device__ __forceinline__ void guard_perm(const PickEval<NCH>& pe, uint32_t& pl
 // no poll limit of its own, so a launch may run as long as its work needs:
 // it ends because every transport wavefront ends (the work counter is finite,
 // every trajectory is capped at kMaxStepsOff steps and every wait for a reply
-// is bounded by kGuardSpinCap), and each of them adds to *done on exit.
+// is bounded by R.guard_spins), and each of them adds to *done on exit.
 
 template <int NCH>
 __device__ void guard_service(const DevCfg& c, GuardBox* boxes, uint32_t* pending, uint32_t* done, int n_boxes) {
@@ -347,9 +346,9 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
             __hip_atomic_fetch_or(pending, box->bit, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
           uint32_t spins = 0;
-          while (box_load(&box->state) != kBoxReplied && ++spins < kGuardSpinCap) __builtin_amdgcn_s_sleep(2);
+          while (box_load(&box->state) != kBoxReplied && ++spins < R.guard_spins) __builtin_amdgcn_s_sleep(2);
           if (ch < 0) {
-            if (spins < kGuardSpinCap) {
+            if (spins < R.guard_spins) {
               ch = box->lane[lane].meta;
               sflags |= MCD_FLAG_FP64_PICK;
               n_fp64++;
@@ -619,11 +618,13 @@ int batched_cfg_floats() { return kCfgFloats; }
 // register budget.
 template <int MODE, int NCH, int PHASE, bool PERIOD, bool EXPL = false>
 __device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, const float* pick, uint64_t seed,
-                                         uint32_t refill_mask, uint32_t& ne, uint32_t& nf, GuardBox* box,
-                                         uint32_t* pending) {
+                                         uint32_t refill_mask, unsigned long long* ic, uint32_t& ne, uint32_t& nf,
+                                         GuardBox* box, uint32_t* pending) {
   RunArgs R = itR;
   R.seed = seed;
   R.refill_mask = refill_mask;
+  // the item's counter block of this launch: phases 2 and 3 count errors in it
+  if constexpr (PHASE >= 2) R.counters = ic;
   // c is the wavefront's LDS copy, so its two table descriptors are in LDS too
   transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0, PERIOD, EXPL>(c, pick, &c.tm, R, ne, nf, box, pending);
 }
@@ -759,24 +760,26 @@ __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ?
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const DevCfg& c = *reinterpret_cast<const DevCfg*>(my);
+    // set only where the loop uses it, so it is not live across the other phases' loops
+    unsigned long long* icl = PHASE >= 2 ? counters + (size_t)it.slot * MCD_NCOUNTERS : nullptr;
     uint32_t ne = 0, nf = 0;
     if (mode == MCD_MODE_CONT) {
       if (PHASE != 2) {
         if (c.bethe == MCD_BETHE_EXPLICIT)
-          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD, true>(c, it.R, pick, seed, refill_mask, ne, nf, nullptr, nullptr);
+          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD, true>(c, it.R, pick, seed, refill_mask, icl, ne, nf, nullptr, nullptr);
         else
-          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, nullptr, nullptr);
+          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, nullptr, nullptr);
       }
     } else {
       switch (nch) {
-        case 4: run_item<MCD_MODE_DS, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
-        case 5: run_item<MCD_MODE_DS, 5, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
-        case 6: run_item<MCD_MODE_DS, 6, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
-        case 7: run_item<MCD_MODE_DS, 7, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
-        case 8: run_item<MCD_MODE_DS, 8, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
-        case 9: run_item<MCD_MODE_DS, 9, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
-        case 10: run_item<MCD_MODE_DS, 10, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
-        default: run_item<MCD_MODE_DS, 11, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, ne, nf, box, pending); break;
+        case 4: run_item<MCD_MODE_DS, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 5: run_item<MCD_MODE_DS, 5, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 6: run_item<MCD_MODE_DS, 6, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 7: run_item<MCD_MODE_DS, 7, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 8: run_item<MCD_MODE_DS, 8, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 9: run_item<MCD_MODE_DS, 9, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 10: run_item<MCD_MODE_DS, 10, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        default: run_item<MCD_MODE_DS, 11, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
       }
     }
     const uint64_t cnt[2] = {ne, nf};

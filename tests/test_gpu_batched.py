@@ -125,6 +125,33 @@ def test_served_batch_at_several_workgroup_sizes(small_batch):
             assert r.els.tobytes() == refs[i], (block, i)
 
 
+def test_served_batch_guard_timeout_is_reported(small_batch, monkeypatch):
+    """The batched served guard's wait is bounded as well: a batch created
+    with MCD_GUARD_SPIN_CAP=1 still ends normally, counts the steps that gave
+    up and the host refuses its records (GuardError); a batch created with the
+    default bound has no timeouts and the per-configuration records."""
+    from mcd import _abi, engine
+    from mcd.engine import GuardError
+    cfgs, trs = small_batch
+    items = [(tr, c["n"]) for tr, c in zip(trs, cfgs)]
+    monkeypatch.setenv("MCD_GUARD_SPIN_CAP", "1")
+    b = engine.Batch(items, pick_nodes=128)
+    b.launch(seed=3, use_graph=False, guard="service")
+    with pytest.raises(GuardError):
+        b.results()
+    cnt = b.counters.cpu().numpy().view(np.uint64).reshape(len(items), _abi.NCOUNTERS)
+    assert int(cnt[:, _abi.CNT_GUARD_TIMEOUTS].sum()) > 0
+    assert [int(c) for c in cnt[:, _abi.CNT_ELECTRONS]] == [n for _, n in items]
+    monkeypatch.delenv("MCD_GUARD_SPIN_CAP")
+    b = engine.Batch(items, pick_nodes=128)
+    b.launch(seed=3, use_graph=False, guard="service")
+    res = b.results()
+    assert sum(int(r.counters[_abi.CNT_FP64_PICKS]) for r in res) > 0
+    for i, ((tr, n), r) in enumerate(zip(items, res)):
+        assert int(r.counters[_abi.CNT_GUARD_TIMEOUTS]) == 0
+        assert r.els.tobytes() == tr.run(n, seed=3, config_id=i).els.tobytes(), i
+
+
 def test_autotuner_picks_and_caches_a_launch_shape(small_batch, tmp_path):
     """The tuner times every candidate, returns the fastest, caches it by the
     batch's configurations, and the tuned batch reproduces per-configuration

@@ -612,6 +612,35 @@ def test_service_guard_other_channel_counts(material, E0, Emin):
     assert int(a.counters[_abi.CNT_FP64_PICKS]) == int(b.counters[_abi.CNT_FP64_PICKS])
 
 
+def test_served_guard_timeout_is_reported(cu20, monkeypatch):
+    """The served guard's wait for a reply is bounded; driven past the bound
+    from the host (MCD_GUARD_SPIN_CAP=1: a transport wavefront polls its
+    mailbox once), the launch still ends normally, the steps that gave up are
+    flagged and counted, and the host refuses the records (GuardError).  With
+    the default bound the same launch has no timeouts and the inline form's
+    records."""
+    import torch
+    from mcd.engine import GuardError
+    tr = cu20[0]
+    n = 1 << 16
+    monkeypatch.setenv("MCD_GUARD_SPIN_CAP", "1")
+    with pytest.raises(GuardError):
+        tr.run(n, seed=1, guard="service")
+    dev = torch.device("cuda", 0)
+    els = torch.empty(n * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    cnt = torch.zeros(_abi.NCOUNTERS, dtype=torch.int64, device=dev)
+    tr.launch(n, els, cnt, seed=1, guard="service")
+    torch.cuda.synchronize(dev)
+    c = cnt.cpu().numpy().view(np.uint64)
+    rec = els.cpu().numpy().view(_abi.ELECTRON_DTYPE)
+    assert int(c[_abi.CNT_GUARD_TIMEOUTS]) > 0 and int(c[_abi.CNT_ELECTRONS]) == n
+    assert ((rec["flags"] & _abi.FLAG_GUARD_TIMEOUT) != 0).any()
+    monkeypatch.delenv("MCD_GUARD_SPIN_CAP")
+    ok = tr.run(n, seed=1, guard="service")
+    assert int(ok.counters[_abi.CNT_GUARD_TIMEOUTS]) == 0
+    assert ok.els.tobytes() == tr.run(n, seed=1, guard="inline").els.tobytes()
+
+
 def test_launch_shape_errors_are_reported(cu20):
     """The C-ABI rejects two guard placements at once and out-of-range shapes
     with MCD_ERR_INVALID before launching anything."""
