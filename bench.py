@@ -238,6 +238,33 @@ def batched_ratio(reps=3, tune=True):
     trs = workloads.build_transports(cfgs, batch_tables=True)
     torch.cuda.synchronize(dev)
     t_batch_tables = time.perf_counter() - t
+    # the same batched build with the grid searches on host threads
+    os.environ["MCD_GRID_SEARCH"] = "host"
+    try:
+        t = time.perf_counter()
+        trs_host = workloads.build_transports(cfgs, batch_tables=True)
+        torch.cuda.synchronize(dev)
+        t_batch_host = time.perf_counter() - t
+    finally:
+        os.environ.pop("MCD_GRID_SEARCH", None)
+    del trs_host
+    # the grid searches alone: every DS table in one device launch
+    # (mcd_grid_search_batch) against the host search of each on one core
+    from mcd.material import Material
+    from mcd.tables import gryz_grid, moller_grid
+    grids = []
+    for c in cfgs:
+        if c["mode"] == "DS":
+            m = Material(c["material"])
+            grids += [moller_grid(m, c["E0"], c["Emin"], c["Wc"]), gryz_grid(m, c["E0"], c["Emin"])]
+    tol = cfgs[0]["tol"]
+    t = time.perf_counter()
+    engine.grid_search_batch(grids, tol, tol)
+    t_search_dev = time.perf_counter() - t
+    t = time.perf_counter()
+    for g in grids:
+        g.build_grids_library(tol, tol)
+    t_search_host = time.perf_counter() - t
     items = [(tr, c["n"]) for tr, c in zip(trs, cfgs)]
     els = [torch.empty(c["n"] * _abi.ELECTRON_DTYPE.itemsize, dtype=torch.uint8, device=dev) for c in cfgs]
     cnt = torch.zeros(len(cfgs) * _abi.NCOUNTERS, dtype=torch.int64, device=dev)
@@ -269,9 +296,13 @@ def batched_ratio(reps=3, tune=True):
                         f"{sum(c['n'] for c in cfgs)} electrons (log-uniform 2^14..2^20, seed 3)",
             "sequential_ms": t_seq, "batched_ms": t_bat, "ratio": t_seq / t_bat,
             "tables": {"per_configuration_ms": t_one * 1e3, "batched_ms": t_batch_tables * 1e3,
+                       "batched_host_search_ms": t_batch_host * 1e3,
+                       "grid_search": {"tables": len(grids), "device_one_launch_ms": t_search_dev * 1e3,
+                                       "host_one_core_ms": t_search_host * 1e3},
                        "note": "DS tables of the 64 configurations from tol_E = tol_W = 1e-3: grid search, FP64 "
                                "CDF rows, row records and channel-pick tables; mcd_tables_build per configuration "
-                               "vs one mcd_tables_build_batch"},
+                               "vs one mcd_tables_build_batch (grid searches in one device launch; with "
+                               "MCD_GRID_SEARCH=host on host threads)"},
             "tuned": {"block": tuned["block"], "pick_nodes": tuned["pick_nodes"], "guard": tuned["guard"],
                       "refill_period": tuned.get("refill_period", 1), "candidates_ms": tuned["ms"]}}
 

@@ -306,6 +306,26 @@ int mcd_tables_set_pick_nodes(mcd_ctx* ctx, mcd_tables* t, int n_nodes);
  * NULL returns the sizes only. */
 int mcd_grid_search(int kind, double Wmin, double n_el, double Eb, double Ef, double Emin, double Emax,
                     double tol_E, double tol_W, float* Es, int32_t* nE, float* Ws, int32_t* nW);
+/* One table's search parameters for mcd_grid_search_batch (the arguments of
+ * mcd_grid_search). */
+typedef struct mcd_grid_spec {
+  int32_t kind; /* MCD_TABLE_MOLLER or MCD_TABLE_GRYZ */
+  int32_t pad;
+  double Wmin, n_el, Eb, Ef, Emin, Emax;
+} mcd_grid_spec;
+/* mcd_grid_search for n tables in one device launch (findWseries /
+ * findEseries, MC/tools/probTables.py:280-382, for every table of a batch):
+ * one workgroup per table runs the search's steps in order, the same
+ * candidate sizes tested in the same order against the same FP64
+ * Gauss-Kronrod reference integral, sums formed in numpy's pairwise order;
+ * the sizes equal mcd_grid_search's, and the grids follow from them
+ * (geomspace for W, linspace for E, as mcd_grid_search returns them).
+ * nE[i], nW[i]: table i's sizes.  status (n entries, may be NULL): MCD_OK or
+ * MCD_ERR_TABLE per table (no grid meets the tolerance, empty W range, or a
+ * loss grid reaching the incident energy); with status NULL a failing table
+ * fails the call with a message naming it. */
+int mcd_grid_search_batch(mcd_ctx* ctx, const mcd_grid_spec* specs, int n, double tol_E, double tol_W, int32_t* nE,
+                          int32_t* nW, int32_t* status);
 /* Tables of a configuration from its input-file parameters (probTables.genTables,
  * MC/tools/probTables.py:541-614): the grid search above for the Moller table
  * and the outermost shell's Gryzinski table (the one every Gryzinski event
@@ -315,9 +335,10 @@ int mcd_grid_search(int kind, double Wmin, double n_el, double Eb, double Ef, do
 int mcd_tables_build(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, int precision,
                      mcd_tables** out);
 /* mcd_tables_build for n configurations at once (genTables, probTables.py:541-614,
- * for every configuration of a batch): the grid searches run on up to 16 host
- * threads, then one device launch fills the CDF rows of every table and one
- * more their row records.  out[i] receives configuration i's tables, equal bit
+ * for every configuration of a batch): the grid searches of every table run
+ * in one device launch (mcd_grid_search_batch; MCD_GRID_SEARCH=host in the
+ * environment runs them on up to 16 host threads instead), then one device
+ * launch fills the CDF rows of every table and one more their row records.  out[i] receives configuration i's tables, equal bit
  * for bit to mcd_tables_build's; on an error every out[i] is NULL and the
  * message names the configuration. */
 int mcd_tables_build_batch(mcd_ctx* ctx, const mcd_config* cfgs, int n, double tol_E, double tol_W, int precision,

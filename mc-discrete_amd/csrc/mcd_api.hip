@@ -543,30 +543,59 @@ extern "C" int mcd_tables_download(mcd_ctx* ctx, const mcd_tables* t, int which,
 // The two grids of a DS configuration; no context, so searches of several
 // configurations may run on separate threads.  err receives the reason of a
 // failure.
-static int config_grids(const mcd_config* cfg, double tol_E, double tol_W, std::vector<float>* E,
-                        std::vector<float>* W, mcd_table_grid* g, char* err, size_t err_len) {
+// The two table searches of a DS configuration: Moller, losses over
+// [Wc, W_max(E0)] and energies over [Emin, E0] (probTables.py:557-575); the
+// outermost shell's Gryzinski table, the one every Gryzinski event reads
+// (probTables.py:579, scattering.py:625), energies from max(Emin, E_b).
+static void config_grid_jobs(const mcd_config* cfg, GridSearchJob j[2]) {
   const mcd_material& m = cfg->mat;
-  char why[200];
-  // Moller: losses over [Wc, W_max(E0)], energies over [Emin, E0] (probTables.py:557-575)
-  int rc = search_grid(MCD_TABLE_MOLLER, cfg->Wc, m.n_val, 0.0, m.fermi_e, cfg->Emin, cfg->E0, tol_E, tol_W, E[0],
-                       W[0], why, sizeof why);
-  if (rc) {
-    snprintf(err, err_len, "Moller grid search: %s", why);
-    return rc;
-  }
-  // Gryzinski: the outermost shell, the table every Gryzinski event reads
-  // (probTables.py:579, scattering.py:625); energies from max(Emin, E_b)
+  j[0] = {MCD_TABLE_MOLLER, cfg->Wc, m.n_val, 0.0, m.fermi_e, cfg->Emin, cfg->E0};
   const int sh = m.n_shells - 1;
   const double Eb = m.shell_Eb[sh], ns = m.shell_n[sh];
-  const double lo = cfg->Emin < Eb ? Eb : cfg->Emin;
-  rc = search_grid(MCD_TABLE_GRYZ, Eb, ns, Eb, m.fermi_e, lo, cfg->E0, tol_E, tol_W, E[1], W[1], why, sizeof why);
-  if (rc) {
-    snprintf(err, err_len, "Gryzinski grid search: %s", why);
-    return rc;
+  j[1] = {MCD_TABLE_GRYZ, Eb, ns, Eb, m.fermi_e, cfg->Emin < Eb ? Eb : cfg->Emin, cfg->E0};
+}
+
+static void grid_of_job(const GridSearchJob& j, std::vector<float>& E, std::vector<float>& W, mcd_table_grid* g) {
+  *g = {(int32_t)E.size(), (int32_t)W.size(), E.data(), W.data(), j.Wmin, j.n_el, j.Eb, j.kind};
+}
+
+static int config_grids(const mcd_config* cfg, double tol_E, double tol_W, std::vector<float>* E,
+                        std::vector<float>* W, mcd_table_grid* g, char* err, size_t err_len) {
+  GridSearchJob j[2];
+  config_grid_jobs(cfg, j);
+  char why[200];
+  for (int q = 0; q < 2; ++q) {
+    const int rc = search_grid(j[q].kind, j[q].Wmin, j[q].n_el, j[q].Eb, j[q].Ef, j[q].Emin, j[q].Emax, tol_E, tol_W,
+                               E[q], W[q], why, sizeof why);
+    if (rc) {
+      snprintf(err, err_len, "%s grid search: %s", q == 0 ? "Moller" : "Gryzinski", why);
+      return rc;
+    }
   }
-  g[0] = {(int32_t)E[0].size(), (int32_t)W[0].size(), E[0].data(), W[0].data(), cfg->Wc, m.n_val, 0.0,
-          MCD_TABLE_MOLLER};
-  g[1] = {(int32_t)E[1].size(), (int32_t)W[1].size(), E[1].data(), W[1].data(), Eb, ns, Eb, MCD_TABLE_GRYZ};
+  for (int q = 0; q < 2; ++q) grid_of_job(j[q], E[q], W[q], &g[q]);
+  return MCD_OK;
+}
+
+// The device search of n tables (one launch); res[i] per job.
+static int device_grid_search(mcd_ctx* ctx, const std::vector<GridSearchJob>& jobs, double tol_E, double tol_W,
+                              std::vector<GridSearchResult>& res) {
+  const int n = (int)jobs.size();
+  res.assign(n, GridSearchResult{0, 0, 0, 0, 0.});
+  if (n == 0) return MCD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  GridSearchJob* d_jobs = nullptr;
+  GridSearchResult* d_res = nullptr;
+  void* d_scr = nullptr;
+  hipError_t e = hipMalloc(&d_jobs, n * sizeof(GridSearchJob));
+  if (e == hipSuccess) e = hipMalloc(&d_res, n * sizeof(GridSearchResult));
+  if (e == hipSuccess) e = hipMalloc(&d_scr, grid_search_scratch_bytes(n));
+  if (e == hipSuccess) e = hipMemcpy(d_jobs, jobs.data(), n * sizeof(GridSearchJob), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_grid_search(d_jobs, n, tol_E, tol_W, d_scr, d_res, 0);
+  if (e == hipSuccess) e = hipMemcpy(res.data(), d_res, n * sizeof(GridSearchResult), hipMemcpyDeviceToHost);
+  (void)hipFree(d_jobs);
+  (void)hipFree(d_res);
+  (void)hipFree(d_scr);
+  if (e != hipSuccess) return set_err(ctx, MCD_ERR_DEVICE, "device grid search: %s", hipGetErrorString(e));
   return MCD_OK;
 }
 
@@ -602,6 +631,38 @@ extern "C" int mcd_grid_search(int kind, double Wmin, double n_el, double Eb, do
   return MCD_OK;
 }
 
+extern "C" int mcd_grid_search_batch(mcd_ctx* ctx, const mcd_grid_spec* specs, int n, double tol_E, double tol_W,
+                                     int32_t* nE, int32_t* nW, int32_t* status) {
+  if (!ctx || (n > 0 && (!specs || !nE || !nW)) || n < 0)
+    return set_err(ctx, MCD_ERR_INVALID, "mcd_grid_search_batch: NULL argument");
+  if (!(tol_E > 0.0 && tol_E < 1.0) || !(tol_W > 0.0 && tol_W < 1.0))
+    return set_err(ctx, MCD_ERR_INVALID, "tolerances tol_E = %g, tol_W = %g outside (0, 1)", tol_E, tol_W);
+  std::vector<GridSearchJob> jobs(n);
+  for (int i = 0; i < n; ++i) {
+    const mcd_grid_spec& p = specs[i];
+    if (p.kind != MCD_TABLE_MOLLER && p.kind != MCD_TABLE_GRYZ)
+      return set_err(ctx, MCD_ERR_INVALID, "table %d: unknown table kind %d", i, p.kind);
+    if (!(p.Wmin > 0.0) || !(p.Emax > p.Emin) || !(p.Emin > 0.0))
+      return set_err(ctx, MCD_ERR_INVALID, "table %d: invalid range", i);
+    jobs[i] = {p.kind, p.Wmin, p.n_el, p.Eb, p.Ef, p.Emin, p.Emax};
+  }
+  std::vector<GridSearchResult> res;
+  int rc = device_grid_search(ctx, jobs, tol_E, tol_W, res);
+  if (rc) return rc;
+  int first_bad = -1;
+  for (int i = 0; i < n; ++i) {
+    nE[i] = res[i].nE;
+    nW[i] = res[i].nW;
+    if (status) status[i] = res[i].status ? MCD_ERR_TABLE : MCD_OK;
+    if (res[i].status && first_bad < 0) first_bad = i;
+  }
+  if (first_bad >= 0) {
+    set_err(ctx, MCD_ERR_TABLE, "table %d: %s", first_bad, grid_status_text(res[first_bad].status));
+    if (!status) return MCD_ERR_TABLE;
+  }
+  return MCD_OK;
+}
+
 extern "C" int mcd_tables_build(mcd_ctx* ctx, const mcd_config* cfg, double tol_E, double tol_W, int precision,
                                 mcd_tables** out) {
   if (!ctx || !cfg || !out) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_build: NULL argument");
@@ -626,8 +687,9 @@ extern "C" int mcd_tables_build(mcd_ctx* ctx, const mcd_config* cfg, double tol_
   return MCD_OK;
 }
 
-// Tables of many configurations: the grid searches on host threads, then the
-// CDF rows of every table in one launch and their row records in another.
+// Tables of many configurations: the grid searches of every table in one
+// device launch (on host threads with MCD_GRID_SEARCH=host), then the CDF rows
+// of every table in one launch and their row records in another.
 extern "C" int mcd_tables_build_batch(mcd_ctx* ctx, const mcd_config* cfgs, int n, double tol_E, double tol_W,
                                       int precision, mcd_tables** out) {
   if (!ctx || !cfgs || !out || n < 0) return set_err(ctx, MCD_ERR_INVALID, "mcd_tables_build_batch: NULL argument");
@@ -645,27 +707,56 @@ extern "C" int mcd_tables_build_batch(mcd_ctx* ctx, const mcd_config* cfgs, int 
     }
   }
   if (n == 0) return MCD_OK;
-  // grid searches: configurations dealt to threads round-robin
   std::vector<std::array<std::vector<float>, 2>> E(n), W(n);
   std::vector<std::array<mcd_table_grid, 2>> g(n);
-  std::vector<int> rcs(n, MCD_OK);
-  std::vector<std::array<char, 256>> errs(n);
-  unsigned hw = std::thread::hardware_concurrency();
-  const int n_threads = (int)std::max(1u, std::min({hw ? hw : 1u, 16u, (unsigned)n}));
-  auto work = [&](int tid) {
-    for (int i = tid; i < n; i += n_threads)
-      if (cfgs[i].mode == MCD_MODE_DS)
-        rcs[i] = config_grids(&cfgs[i], tol_E, tol_W, E[i].data(), W[i].data(), g[i].data(), errs[i].data(),
-                              errs[i].size());
-  };
-  {
-    std::vector<std::thread> pool;
-    for (int t = 1; t < n_threads; ++t) pool.emplace_back(work, t);
-    work(0);
-    for (std::thread& th : pool) th.join();
+  const char* where = getenv("MCD_GRID_SEARCH");
+  if (where && !strcmp(where, "host")) {
+    // grid searches on host threads, configurations dealt round-robin
+    std::vector<int> rcs(n, MCD_OK);
+    std::vector<std::array<char, 256>> errs(n);
+    unsigned hw = std::thread::hardware_concurrency();
+    const int n_threads = (int)std::max(1u, std::min({hw ? hw : 1u, 16u, (unsigned)n}));
+    auto work = [&](int tid) {
+      for (int i = tid; i < n; i += n_threads)
+        if (cfgs[i].mode == MCD_MODE_DS)
+          rcs[i] = config_grids(&cfgs[i], tol_E, tol_W, E[i].data(), W[i].data(), g[i].data(), errs[i].data(),
+                                errs[i].size());
+    };
+    {
+      std::vector<std::thread> pool;
+      for (int t = 1; t < n_threads; ++t) pool.emplace_back(work, t);
+      work(0);
+      for (std::thread& th : pool) th.join();
+    }
+    for (int i = 0; i < n; ++i)
+      if (rcs[i]) return set_err(ctx, rcs[i], "configuration %d: %s", i, errs[i].data());
+  } else {
+    // every table's search in one device launch, the grids from their sizes
+    std::vector<GridSearchJob> jobs;
+    std::vector<int> of;
+    for (int i = 0; i < n; ++i)
+      if (cfgs[i].mode == MCD_MODE_DS) {
+        GridSearchJob j[2];
+        config_grid_jobs(&cfgs[i], j);
+        jobs.push_back(j[0]);
+        jobs.push_back(j[1]);
+        of.push_back(i);
+      }
+    std::vector<GridSearchResult> res;
+    int rc = device_grid_search(ctx, jobs, tol_E, tol_W, res);
+    if (rc) return rc;
+    for (size_t k = 0; k < of.size(); ++k) {
+      const int i = of[k];
+      for (int q = 0; q < 2; ++q) {
+        const GridSearchResult& r = res[2 * k + q];
+        if (r.status)
+          return set_err(ctx, MCD_ERR_TABLE, "configuration %d: %s grid search: %s", i,
+                         q == 0 ? "Moller" : "Gryzinski", grid_status_text(r.status));
+        grid_materialize(jobs[2 * k + q], r.nE, r.nW, E[i][q], W[i][q]);
+        grid_of_job(jobs[2 * k + q], E[i][q], W[i][q], &g[i][q]);
+      }
+    }
   }
-  for (int i = 0; i < n; ++i)
-    if (rcs[i]) return set_err(ctx, rcs[i], "configuration %d: %s", i, errs[i].data());
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   auto fail = [&](int rc) {
     for (int i = 0; i < n; ++i)

@@ -152,4 +152,28 @@ int search_grid(int kind, double Wmin, double n_el, double Eb, double Ef, double
 // The reference integral of the search: dsigma/dW over [Wmin, (E - Ef)/2] at E.
 double grid_reference_integral(int kind, double Wmin, double n_el, double Eb, double Ef, double E);
 
+// The same search on the device for many tables (mcd_grid.hip): one
+// workgroup per table; results are the grid sizes (the grids follow from
+// them, grid_materialize) and the reference integral.
+struct GridSearchJob {
+  int kind;
+  double Wmin, n_el, Eb, Ef, Emin, Emax;
+};
+struct GridSearchResult {
+  int32_t nE, nW, status, pad;
+  double ref;
+};
+constexpr int kGridOrders = 5, kGridPercents = 10;
+// candidate sizes: ordersOfMag(1e6) and percents(n) of each (probTables.py:54-64)
+struct GridCandidates {
+  int32_t orders[kGridOrders];
+  int32_t pct[kGridOrders][kGridPercents];
+};
+enum { kGridEmptyW = 1, kGridNoW, kGridWRefine, kGridWReachesE, kGridNoE, kGridERefine };
+size_t grid_search_scratch_bytes(int n_jobs);
+hipError_t launch_grid_search(const GridSearchJob* jobs, int n_jobs, double tol_E, double tol_W, void* scratch,
+                              GridSearchResult* res, hipStream_t s);
+void grid_materialize(const GridSearchJob& j, int nE, int nW, std::vector<float>& Es, std::vector<float>& Ws);
+const char* grid_status_text(int status);
+
 }  // namespace mcd

@@ -57,6 +57,12 @@ class TableGridC(C.Structure):
     ]
 
 
+class GridSpec(C.Structure):
+    """mcd_grid_spec: one table's grid-search parameters."""
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("Wmin", C.c_double), ("n_el", C.c_double),
+                ("Eb", C.c_double), ("Ef", C.c_double), ("Emin", C.c_double), ("Emax", C.c_double)]
+
+
 class ScatterBuf(C.Structure):
     _fields_ = [("rows", C.c_void_p), ("capacity", C.c_uint64), ("max_electron", C.c_uint64)]
 

@@ -65,6 +65,8 @@ def lib():
     L.mcd_scatter_pack.restype = i32
     L.mcd_scatter_pack.argtypes = [vp, vp, C.c_uint64, C.c_uint32, vp, C.c_size_t, P(C.c_size_t), vp]
     L.mcd_tables_build_batch.argtypes = [vp, P(_abi.Config), i32, f64, f64, i32, P(vp)]
+    L.mcd_grid_search_batch.restype = i32
+    L.mcd_grid_search_batch.argtypes = [vp, P(_abi.GridSpec), i32, f64, f64, P(i32), P(i32), P(i32)]
     L.mcd_tables_grid.restype = i32
     L.mcd_tables_grid.argtypes = [vp, vp, i32, P(i32), P(i32), vp, vp]
     L.mcd_tables_save.restype = i32

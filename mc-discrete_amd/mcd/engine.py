@@ -453,6 +453,27 @@ class Batch:
         return out
 
 
+def grid_search_batch(grids, tol_E, tol_W, device=0, statuses=False):
+    """Grid sizes of many tables from one device launch (mcd_grid_search_batch,
+    findWseries / findEseries, probTables.py:280-382).  grids: TableGrid
+    objects (mcd.tables.moller_grid / gryz_grid).  Returns int arrays nE, nW
+    and, with statuses=True, the per-table status (0 or ERR_TABLE) instead of
+    raising on the first table no grid can serve."""
+    n = len(grids)
+    arr = (_abi.GridSpec * max(n, 1))()
+    for i, g in enumerate(grids):
+        arr[i] = _abi.GridSpec(0 if g.kind == "Moller" else 1, 0, float(g.Wmin), float(g.n_el), float(g.Eb),
+                               float(g.Ef), float(g.Emin), float(g.Emax))
+    nE = np.zeros(max(n, 1), np.int32)
+    nW = np.zeros(max(n, 1), np.int32)
+    st = np.zeros(max(n, 1), np.int32)
+    P = C.POINTER(C.c_int32)
+    ctx = context(device)
+    check(lib().mcd_grid_search_batch(ctx, arr, n, float(tol_E), float(tol_W), nE.ctypes.data_as(P),
+                                      nW.ctypes.data_as(P), st.ctypes.data_as(P) if statuses else None), ctx)
+    return (nE[:n], nW[:n], st[:n]) if statuses else (nE[:n], nW[:n])
+
+
 def build_transports_batch(specs, tol_E=1e-3, tol_W=1e-3, table_precision=_abi.PREC_FP64, device=0, pick_nodes=0):
     """Transports for many configurations with their tables built together
     (mcd_tables_build_batch: grid searches on host threads, the CDF rows and
@@ -487,5 +508,5 @@ def grids_for(mat: Material, E0, Emin, Wc, tol_E, tol_W):
 
 
 __all__ = ["Transport", "Batch", "RunResult", "context", "grids_for", "TableGrid", "backscatter_stats",
-           "build_transports_batch",
+           "build_transports_batch", "grid_search_batch",
            "GuardError", "check_guard"]

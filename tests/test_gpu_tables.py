@@ -118,14 +118,58 @@ def test_gen_tables_no_reads_the_keyed_file(tmp_path):
     assert len(os.listdir(out / "tables")) == 2
 
 
+def test_device_grid_search_equals_host_search():
+    """mcd_grid_search_batch (one device launch, a workgroup per table) picks
+    the grid sizes the host search (mcd_grid_search, itself equal bit for bit
+    to the scipy-based restatement) picks, for the 44 tables of the host
+    search's test cases: C1, C2, the reference's test input at tol_W 1e-7, the
+    Wc range ends and the batched workload's DS tables, one launch per
+    tolerance pair."""
+    from mcd.engine import grid_search_batch
+    from mcd.material import Material
+    from mcd.tables import gryz_grid, moller_grid
+    from test_table_builder import CASES
+    groups = {}
+    for mat, E0, Emin, Wc, tE, tW in CASES:
+        m = Material(mat)
+        groups.setdefault((tE, tW), []).extend([moller_grid(m, E0, Emin, Wc), gryz_grid(m, E0, Emin)])
+    for (tE, tW), grids in groups.items():
+        nE, nW, st = grid_search_batch(grids, tE, tW, statuses=True)
+        assert not st.any()
+        for g, e, w in zip(grids, nE, nW):
+            h = g.build_grids_library(tE, tW)
+            assert (int(e), int(w)) == (h.Es.size, h.Ws.size), (g.kind, g.Emax, g.Wmin, tE, tW)
+
+
+def test_device_grid_search_reports_each_table():
+    """A table no grid can serve (a Moller table whose loss range is empty:
+    Wc above W_max(E0)) gets ERR_TABLE in its own status slot while the others
+    are searched; without the status array the call fails naming it."""
+    from mcd import _abi
+    from mcd._lib import MCDError
+    from mcd.engine import grid_search_batch
+    from mcd.material import Material
+    from mcd.tables import gryz_grid, moller_grid
+    si = Material("Si")
+    grids = [moller_grid(si, 10000.0, 1000.0, 100.0), moller_grid(si, 1000.0, 500.0, 600.0),
+             gryz_grid(si, 10000.0, 1000.0)]
+    nE, nW, st = grid_search_batch(grids, 1e-2, 1e-2, statuses=True)
+    assert list(st) == [0, _abi.ERR_TABLE, 0]
+    assert (int(nE[0]), int(nW[0]) - 1) == (300, 29) and (int(nE[2]), int(nW[2]) - 1) == (400, 39)
+    with pytest.raises(MCDError) as e:
+        grid_search_batch(grids, 1e-2, 1e-2)
+    assert e.value.code == _abi.ERR_TABLE and "table 1" in str(e.value)
+
+
 def test_batched_build_equals_per_configuration_builds():
-    """mcd_tables_build_batch (grid searches on host threads, one device launch
-    for the CDF rows of every table and one for their row records) gives each
-    configuration the tables mcd_tables_build gives it: grids, CDFs and
-    channel-pick tables bit for bit, and the same records; continuous
-    configurations in the batch get their tables too.  The 64-configuration
-    workload's DS tables at tol 1e-3, in FP64 and in the reference's float32
-    arithmetic."""
+    """mcd_tables_build_batch (every table's grid search in one device launch,
+    one launch for the CDF rows of every table and one for their row records)
+    gives each configuration the tables mcd_tables_build (host grid search)
+    gives it: grids, CDFs and channel-pick tables bit for bit, and the same
+    records; continuous configurations in the batch get their tables too.  The
+    64-configuration workload's DS tables at tol 1e-3, in FP64 and in the
+    reference's float32 arithmetic; the same with the searches on host threads
+    (MCD_GRID_SEARCH=host)."""
     import torch
     from mcd import _abi, workloads
     from mcd.engine import Transport, build_transports_batch
@@ -135,8 +179,13 @@ def test_batched_build_equals_per_configuration_builds():
              for c in ds]
     specs.append(dict(material="Au", E0=30000.0, Emin=1000.0, s_tilt=0.0, mode="cont", bethe_model="JL"))
     dev = torch.device("cuda", 0)
-    for prec in (_abi.PREC_FP64, _abi.PREC_FP32):
-        batch = build_transports_batch(specs, tol_E=1e-3, tol_W=1e-3, table_precision=prec)
+    for prec, where in ((_abi.PREC_FP64, None), (_abi.PREC_FP32, None), (_abi.PREC_FP64, "host")):
+        if where:
+            os.environ["MCD_GRID_SEARCH"] = where
+        try:
+            batch = build_transports_batch(specs, tol_E=1e-3, tol_W=1e-3, table_precision=prec)
+        finally:
+            os.environ.pop("MCD_GRID_SEARCH", None)
         assert len(batch) == len(specs)
         for sp, b in zip(specs, batch):
             one = Transport(tol_E=1e-3, tol_W=1e-3, table_precision=prec, grid_search="library", **sp)
