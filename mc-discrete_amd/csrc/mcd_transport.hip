@@ -615,18 +615,19 @@ int batched_cfg_floats() { return kCfgFloats; }
 // PERIOD: idle lanes take new electrons every refill_mask + 1 steps (the
 // launch's compaction period), and at once when all are idle; otherwise every
 // step.  A separate instantiation, so the every-step kernels keep their
-// register budget.
+// register budget.  c: the item's constants (scalar loads); cl: the
+// wavefront's LDS copy, whose two table descriptors the lanes select from.
 template <int MODE, int NCH, int PHASE, bool PERIOD, bool EXPL = false>
-__device__ __forceinline__ void run_item(const DevCfg& c, const RunArgs& itR, const float* pick, uint64_t seed,
-                                         uint32_t refill_mask, unsigned long long* ic, uint32_t& ne, uint32_t& nf,
-                                         GuardBox* box, uint32_t* pending) {
+__device__ __forceinline__ void run_item(const DevCfg& c, const DevCfg& cl, const RunArgs& itR, const float* pick,
+                                         uint64_t seed, uint32_t refill_mask, unsigned long long* ic, uint32_t& ne,
+                                         uint32_t& nf, GuardBox* box, uint32_t* pending) {
   RunArgs R = itR;
   R.seed = seed;
   R.refill_mask = refill_mask;
   // the item's counter block of this launch: phases 2 and 3 count errors in it
   if constexpr (PHASE >= 2) R.counters = ic;
-  // c is the wavefront's LDS copy, so its two table descriptors are in LDS too
-  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0, PERIOD, EXPL>(c, pick, &c.tm, R, ne, nf, box, pending);
+  // the table descriptors from the LDS copy: the per-lane Moller / Gryzinski select reads LDS
+  transport_loop<MODE, NCH, MODE == MCD_MODE_DS ? PHASE : 0, PERIOD, EXPL>(c, pick, &cl.tm, R, ne, nf, box, pending);
 }
 
 // The batched kernel's service wavefront: as guard_service, for mailboxes
@@ -692,13 +693,20 @@ static size_t batched_lds_bytes(int phase, int block, int slice_floats) {
 // in the kernel).  PHASE 2: DS items resume their parked electrons.  PHASE 3:
 // wavefront 0 of each workgroup serves the guard picks of the others (no FP64
 // code on the transport path, one phase).
-// Register budgets measured on MI355X for the 64-configuration batch: 4
-// resident wavefronts per SIMD for phases 0 and 2, 5 for phase 1 (the FP64-free
-// phase), each faster than the compiler's own allocation despite small spills.
+// The items are read through the constant address space: they do not change
+// during the launch, so the item's configuration constants, the same for the
+// whole wavefront, are scalar loads into SGPRs (reloaded where needed) rather
+// than per-lane LDS reads held in VGPRs.
+// Register budgets measured on MI355X for the 64-configuration batch: 5
+// resident wavefronts per SIMD for phases 0, 1 and 3 (phase 0 spills at its
+// refill, FP64-guard and item-switch code and still beats 4 and 6
+// wavefronts), 4 for phase 2.
 template <int PHASE, bool PERIOD>
-__global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ? 5 : 4) void k_transport_batched(
-    const BatchItemDev* __restrict__ items, int n_items, const int* __restrict__ group_start, int n_groups,
-    int slice_floats, uint64_t seed, uint32_t refill_mask, unsigned long long* __restrict__ counters) {
+__global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 2 ? 4 : 5) void k_transport_batched(
+    const __attribute__((address_space(4))) BatchItemDev* __restrict__ citems, int n_items,
+    const int* __restrict__ group_start, int n_groups, int slice_floats, uint64_t seed, uint32_t refill_mask,
+    unsigned long long* __restrict__ counters) {
+  const BatchItemDev* items = (const BatchItemDev*)citems;
   extern __shared__ float lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -760,26 +768,27 @@ __global__ __launch_bounds__(PHASE == 3 ? 1024 : 256, PHASE == 1 || PHASE == 3 ?
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const DevCfg& c = *reinterpret_cast<const DevCfg*>(my);
+    const DevCfg& cg = (const DevCfg&)citems[ci].c;  // the same constants as scalar loads
     // set only where the loop uses it, so it is not live across the other phases' loops
     unsigned long long* icl = PHASE >= 2 ? counters + (size_t)it.slot * MCD_NCOUNTERS : nullptr;
     uint32_t ne = 0, nf = 0;
     if (mode == MCD_MODE_CONT) {
       if (PHASE != 2) {
         if (c.bethe == MCD_BETHE_EXPLICIT)
-          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD, true>(c, it.R, pick, seed, refill_mask, icl, ne, nf, nullptr, nullptr);
+          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD, true>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, nullptr, nullptr);
         else
-          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, nullptr, nullptr);
+          run_item<MCD_MODE_CONT, 4, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, nullptr, nullptr);
       }
     } else {
       switch (nch) {
-        case 4: run_item<MCD_MODE_DS, 4, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
-        case 5: run_item<MCD_MODE_DS, 5, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
-        case 6: run_item<MCD_MODE_DS, 6, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
-        case 7: run_item<MCD_MODE_DS, 7, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
-        case 8: run_item<MCD_MODE_DS, 8, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
-        case 9: run_item<MCD_MODE_DS, 9, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
-        case 10: run_item<MCD_MODE_DS, 10, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
-        default: run_item<MCD_MODE_DS, 11, PHASE, PERIOD>(c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 4: run_item<MCD_MODE_DS, 4, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 5: run_item<MCD_MODE_DS, 5, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 6: run_item<MCD_MODE_DS, 6, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 7: run_item<MCD_MODE_DS, 7, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 8: run_item<MCD_MODE_DS, 8, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 9: run_item<MCD_MODE_DS, 9, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        case 10: run_item<MCD_MODE_DS, 10, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
+        default: run_item<MCD_MODE_DS, 11, PHASE, PERIOD>(cg, c, it.R, pick, seed, refill_mask, icl, ne, nf, box, pending); break;
       }
     }
     const uint64_t cnt[2] = {ne, nf};
