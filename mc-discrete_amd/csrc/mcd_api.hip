@@ -576,21 +576,25 @@ static int config_grids(const mcd_config* cfg, double tol_E, double tol_W, std::
   return MCD_OK;
 }
 
-// The device search of n tables (one launch); res[i] per job.
+// The device search of n tables (one launch per 256 tables: the scratch is
+// about 0.55 MB per table); res[i] per job.
 static int device_grid_search(mcd_ctx* ctx, const std::vector<GridSearchJob>& jobs, double tol_E, double tol_W,
                               std::vector<GridSearchResult>& res) {
   const int n = (int)jobs.size();
   res.assign(n, GridSearchResult{0, 0, 0, 0, 0.});
   if (n == 0) return MCD_OK;
+  constexpr int kChunk = 256;
+  const int m = std::min(n, kChunk);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   GridSearchJob* d_jobs = nullptr;
   GridSearchResult* d_res = nullptr;
   void* d_scr = nullptr;
   hipError_t e = hipMalloc(&d_jobs, n * sizeof(GridSearchJob));
   if (e == hipSuccess) e = hipMalloc(&d_res, n * sizeof(GridSearchResult));
-  if (e == hipSuccess) e = hipMalloc(&d_scr, grid_search_scratch_bytes(n));
+  if (e == hipSuccess) e = hipMalloc(&d_scr, grid_search_scratch_bytes(m));
   if (e == hipSuccess) e = hipMemcpy(d_jobs, jobs.data(), n * sizeof(GridSearchJob), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_grid_search(d_jobs, n, tol_E, tol_W, d_scr, d_res, 0);
+  for (int i = 0; i < n && e == hipSuccess; i += kChunk)
+    e = launch_grid_search(d_jobs + i, std::min(kChunk, n - i), tol_E, tol_W, d_scr, d_res + i, 0);
   if (e == hipSuccess) e = hipMemcpy(res.data(), d_res, n * sizeof(GridSearchResult), hipMemcpyDeviceToHost);
   (void)hipFree(d_jobs);
   (void)hipFree(d_res);

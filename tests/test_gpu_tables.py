@@ -141,6 +141,24 @@ def test_device_grid_search_equals_host_search():
             assert (int(e), int(w)) == (h.Es.size, h.Ws.size), (g.kind, g.Emax, g.Wmin, tE, tW)
 
 
+def test_device_grid_search_many_tables_in_chunks():
+    """More tables than one launch takes (256 per launch): five copies of the
+    batched workload's 64 DS tables give five copies of the 64 tables' sizes."""
+    from mcd import workloads
+    from mcd.engine import grid_search_batch
+    from mcd.material import Material
+    from mcd.tables import gryz_grid, moller_grid
+    grids = []
+    for c in workloads.batched_configs():
+        if c["mode"] == "DS":
+            m = Material(c["material"])
+            grids += [moller_grid(m, c["E0"], c["Emin"], c["Wc"]), gryz_grid(m, c["E0"], c["Emin"])]
+    nE1, nW1 = grid_search_batch(grids, 1e-3, 1e-3)
+    nE, nW = grid_search_batch(grids * 5, 1e-3, 1e-3)
+    assert len(grids) * 5 > 256
+    assert np.array_equal(nE, np.tile(nE1, 5)) and np.array_equal(nW, np.tile(nW1, 5))
+
+
 def test_device_grid_search_reports_each_table():
     """A table no grid can serve (a Moller table whose loss range is empty:
     Wc above W_max(E0)) gets ERR_TABLE in its own status slot while the others
