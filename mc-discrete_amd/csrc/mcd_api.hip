@@ -326,91 +326,118 @@ static void set_meta(float* node, int n_ch, uint32_t swap, bool exact) {
   memcpy(&node[n_ch + 1], &hi, 4);
 }
 
-// Build the LDS pick table for a DS configuration (see mcd_device.h).
-static int build_pick_table(mcd_ctx* ctx, mcd_tables* t, int n_nodes) {
-  const mcd_config& cfg = t->cfg;
-  t->n_ch = cfg.mat.n_shells + 3;
-  t->stride = pick_stride(t->n_ch);
-  t->n_nodes = n_nodes;
-  if (t->nodes) { (void)hipFree(t->nodes); t->nodes = nullptr; }
-  t->lnE0 = log(cfg.Emin);
-  t->dln = (log(cfg.E0) - t->lnE0) / (double)(t->n_nodes - 1);
-  t->delta = 0.f;
-  const size_t nf = (size_t)t->n_nodes * t->stride;
-  if (hipMalloc(&t->nodes, nf * sizeof(float)) != hipSuccess)
-    return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the pick table failed");
-  DevCfg c = make_devcfg(cfg, t, 0);
-  HIP_TRY(ctx, launch_pick_nodes(c, t->n_nodes, t->lnE0, t->dln, t->nodes, t->stride, 0));
-  std::vector<float> h(nf);
-  HIP_TRY(ctx, hipMemcpy(h.data(), t->nodes, nf * sizeof(float), hipMemcpyDeviceToHost));
-  // cell classification from the node orders
-  t->n_swap_cells = t->n_exact_cells = 0;
-  std::vector<char> exact(t->n_nodes, 0);
-  for (int k = 0; k + 1 < t->n_nodes; ++k) {
-    float* a = &h[(size_t)k * t->stride];
-    float* b = &h[(size_t)(k + 1) * t->stride];
-    int ndiff = 0, first = -1;
-    for (int q = 0; q < t->n_ch; ++q)
-      if (label_at(a, t->n_ch, q) != label_at(b, t->n_ch, q)) { if (first < 0) first = q; ++ndiff; }
-    uint32_t swap = kMetaNoSwap;
-    bool ex = false;
-    if (ndiff == 2 && first + 1 < t->n_ch && label_at(a, t->n_ch, first) == label_at(b, t->n_ch, first + 1) &&
-        label_at(a, t->n_ch, first + 1) == label_at(b, t->n_ch, first))
-      swap = (uint32_t)first;
-    else if (ndiff != 0)
-      ex = true;
-    for (int q = 0; q < t->stride; ++q)
-      if (!(h[(size_t)k * t->stride + q] == h[(size_t)k * t->stride + q])) ex = true;  // NaN node
-    if (swap != kMetaNoSwap) t->n_swap_cells++;
-    exact[k] = ex;
-    set_meta(a, t->n_ch, swap, ex);
+// Build the LDS pick tables of n DS configurations (see mcd_device.h): every
+// table's nodes in one pass of launches, the cell classification on the host,
+// then every table's interpolation check into one buffer read back once.
+static int build_pick_tables(mcd_ctx* ctx, mcd_tables* const* ts, const int* n_nodes, int n) {
+  std::vector<std::vector<float>> h(n);
+  for (int i = 0; i < n; ++i) {
+    mcd_tables* t = ts[i];
+    const mcd_config& cfg = t->cfg;
+    t->n_ch = cfg.mat.n_shells + 3;
+    t->stride = pick_stride(t->n_ch);
+    t->n_nodes = n_nodes[i];
+    if (t->nodes) { (void)hipFree(t->nodes); t->nodes = nullptr; }
+    t->lnE0 = log(cfg.Emin);
+    t->dln = (log(cfg.E0) - t->lnE0) / (double)(t->n_nodes - 1);
+    t->delta = 0.f;
+    const size_t nf = (size_t)t->n_nodes * t->stride;
+    if (hipMalloc(&t->nodes, nf * sizeof(float)) != hipSuccess)
+      return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the pick table failed");
+    DevCfg c = make_devcfg(cfg, t, 0);
+    HIP_TRY(ctx, launch_pick_nodes(c, t->n_nodes, t->lnE0, t->dln, t->nodes, t->stride, 0));
   }
-  HIP_TRY(ctx, hipMemcpy(t->nodes, h.data(), nf * sizeof(float), hipMemcpyHostToDevice));
-  // interpolation check against the exact FP64 pick
+  // cell classification from the node orders
+  std::vector<std::vector<char>> exact(n);
+  for (int i = 0; i < n; ++i) {
+    mcd_tables* t = ts[i];
+    const size_t nf = (size_t)t->n_nodes * t->stride;
+    h[i].resize(nf);
+    HIP_TRY(ctx, hipMemcpy(h[i].data(), t->nodes, nf * sizeof(float), hipMemcpyDeviceToHost));
+    t->n_swap_cells = t->n_exact_cells = 0;
+    exact[i].assign(t->n_nodes, 0);
+    for (int k = 0; k + 1 < t->n_nodes; ++k) {
+      float* a = &h[i][(size_t)k * t->stride];
+      float* b = &h[i][(size_t)(k + 1) * t->stride];
+      int ndiff = 0, first = -1;
+      for (int q = 0; q < t->n_ch; ++q)
+        if (label_at(a, t->n_ch, q) != label_at(b, t->n_ch, q)) { if (first < 0) first = q; ++ndiff; }
+      uint32_t swap = kMetaNoSwap;
+      bool ex = false;
+      if (ndiff == 2 && first + 1 < t->n_ch && label_at(a, t->n_ch, first) == label_at(b, t->n_ch, first + 1) &&
+          label_at(a, t->n_ch, first + 1) == label_at(b, t->n_ch, first))
+        swap = (uint32_t)first;
+      else if (ndiff != 0)
+        ex = true;
+      for (int q = 0; q < t->stride; ++q)
+        if (!(h[i][(size_t)k * t->stride + q] == h[i][(size_t)k * t->stride + q])) ex = true;  // NaN node
+      if (swap != kMetaNoSwap) t->n_swap_cells++;
+      exact[i][k] = ex;
+      set_meta(a, t->n_ch, swap, ex);
+    }
+    HIP_TRY(ctx, hipMemcpy(t->nodes, h[i].data(), nf * sizeof(float), hipMemcpyHostToDevice));
+  }
+  // interpolation check against the exact FP64 pick, M points per cell
   const int M = 16;
-  const size_t npts = (size_t)(t->n_nodes - 1) * M;
+  std::vector<size_t> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (size_t)(ts[i]->n_nodes - 1) * M;
+  const size_t npts = off[n];
   float* d_err = nullptr;
   int* d_bad = nullptr;
-  if (hipMalloc(&d_err, 2 * npts * sizeof(float)) != hipSuccess || hipMalloc(&d_bad, npts * sizeof(int)) != hipSuccess)
+  if (hipMalloc(&d_err, 2 * npts * sizeof(float)) != hipSuccess || hipMalloc(&d_bad, npts * sizeof(int)) != hipSuccess) {
+    (void)hipFree(d_err);
     return set_err(ctx, MCD_ERR_NOMEM, "hipMalloc for the pick-table check failed");
-  c = make_devcfg(cfg, t, 0);
-  hipError_t e1 = launch_pick_check(c, M, d_err, d_bad, 0);
+  }
+  hipError_t e1 = hipSuccess;
+  for (int i = 0; i < n && e1 == hipSuccess; ++i) {
+    DevCfg c = make_devcfg(ts[i]->cfg, ts[i], 0);
+    e1 = launch_pick_check(c, M, d_err + 2 * off[i], d_bad + off[i], 0);
+  }
   std::vector<float> herr(2 * npts);
   std::vector<int> hbad(npts);
-  hipError_t e2 = hipMemcpy(herr.data(), d_err, 2 * npts * sizeof(float), hipMemcpyDeviceToHost);
-  hipError_t e3 = hipMemcpy(hbad.data(), d_bad, npts * sizeof(int), hipMemcpyDeviceToHost);
+  hipError_t e2 = e1 == hipSuccess ? hipMemcpy(herr.data(), d_err, 2 * npts * sizeof(float), hipMemcpyDeviceToHost)
+                                   : hipSuccess;
+  hipError_t e3 = e1 == hipSuccess ? hipMemcpy(hbad.data(), d_bad, npts * sizeof(int), hipMemcpyDeviceToHost)
+                                   : hipSuccess;
   (void)hipFree(d_err);
   (void)hipFree(d_bad);
   HIP_TRY(ctx, e1);
   HIP_TRY(ctx, e2);
   HIP_TRY(ctx, e3);
-  double maxerr = 0.;
-  for (int k = 0; k + 1 < t->n_nodes; ++k) {
-    bool bad = exact[k];
-    double ek = 0.;
-    for (int m = 0; m < M; ++m) {
-      const size_t i = (size_t)k * M + m;
-      if (hbad[i]) bad = true;
-      ek = fmax(ek, fmax((double)herr[2 * i], (double)herr[2 * i + 1]));
-      if (!(herr[2 * i] == herr[2 * i]) || !(herr[2 * i + 1] == herr[2 * i + 1])) bad = true;
-    }
-    if (ek > 1e-4) bad = true;  // badly resolved cell: always exact
-    if (bad) {
-      if (!exact[k]) {
-        uint32_t hi;
-        memcpy(&hi, &h[(size_t)k * t->stride + t->n_ch + 1], 4);
-        set_meta(&h[(size_t)k * t->stride], t->n_ch, (hi >> 12) & 15u, true);
+  for (int i = 0; i < n; ++i) {
+    mcd_tables* t = ts[i];
+    const float* err = herr.data() + 2 * off[i];
+    const int* bad_pt = hbad.data() + off[i];
+    double maxerr = 0.;
+    for (int k = 0; k + 1 < t->n_nodes; ++k) {
+      bool bad = exact[i][k];
+      double ek = 0.;
+      for (int m = 0; m < M; ++m) {
+        const size_t j = (size_t)k * M + m;
+        if (bad_pt[j]) bad = true;
+        ek = fmax(ek, fmax((double)err[2 * j], (double)err[2 * j + 1]));
+        if (!(err[2 * j] == err[2 * j]) || !(err[2 * j + 1] == err[2 * j + 1])) bad = true;
       }
-      exact[k] = 1;
-      t->n_exact_cells++;
-    } else {
-      maxerr = fmax(maxerr, ek);
+      if (ek > 1e-4) bad = true;  // badly resolved cell: always exact
+      if (bad) {
+        if (!exact[i][k]) {
+          uint32_t hi;
+          memcpy(&hi, &h[i][(size_t)k * t->stride + t->n_ch + 1], 4);
+          set_meta(&h[i][(size_t)k * t->stride], t->n_ch, (hi >> 12) & 15u, true);
+        }
+        exact[i][k] = 1;
+        t->n_exact_cells++;
+      } else {
+        maxerr = fmax(maxerr, ek);
+      }
     }
+    t->delta = (float)(2.0 * maxerr + 2e-7);
+    HIP_TRY(ctx, hipMemcpy(t->nodes, h[i].data(), h[i].size() * sizeof(float), hipMemcpyHostToDevice));
   }
-  t->delta = (float)(2.0 * maxerr + 2e-7);
-  HIP_TRY(ctx, hipMemcpy(t->nodes, h.data(), nf * sizeof(float), hipMemcpyHostToDevice));
   return MCD_OK;
 }
+
+static int build_pick_table(mcd_ctx* ctx, mcd_tables* t, int n_nodes) { return build_pick_tables(ctx, &t, &n_nodes, 1); }
 
 // New tables for a checked configuration; DS: the grids uploaded and the CDF
 // and row-record buffers allocated, not yet filled.
@@ -802,9 +829,18 @@ extern "C" int mcd_tables_build_batch(mcd_ctx* ctx, const mcd_config* cfgs, int 
     (void)hipFree(d_jobs);
     if (e != hipSuccess) return fail(set_err(ctx, MCD_ERR_DEVICE, "batched table build: %s", hipGetErrorString(e)));
   }
-  for (int i = 0; i < n; ++i) {
-    int rc = tables_finish(ctx, out[i]);
+  {
+    // the channel-pick tables of every DS configuration together
+    std::vector<mcd_tables*> ds;
+    for (int i = 0; i < n; ++i)
+      if (out[i]->has_w) ds.push_back(out[i]);
+    const std::vector<int> nodes(ds.size(), pick_nodes_default());
+    int rc = ds.empty() ? MCD_OK : build_pick_tables(ctx, ds.data(), nodes.data(), (int)ds.size());
     if (rc) return fail(rc);
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(set_err(ctx, MCD_ERR_DEVICE, "table build: %s", hipGetErrorString(e)));
+  }
+  for (int i = 0; i < n; ++i) {
     out[i]->keyed = true;
     out[i]->tol_E = tol_E;
     out[i]->tol_W = tol_W;

@@ -123,8 +123,7 @@ class Transport:
         if tables is not None:
             self._tables = tables
             if self.mode == _abi.MODE_DS:
-                self.grids = (self._library_grid(_abi.TABLE_MOLLER, moller_grid(self.mat, E0, Emin, Wc)),
-                              self._library_grid(_abi.TABLE_GRYZ, gryz_grid(self.mat, E0, Emin)))
+                self._lazy_grids = (E0, Emin, Wc)  # copied from the tables on first use of .grids
                 if pick_nodes:
                     check(lib().mcd_tables_set_pick_nodes(ctx, self._tables, int(pick_nodes)), ctx)
         elif self.mode == _abi.MODE_DS and self.grids is None and cdfs is None and (
@@ -180,6 +179,24 @@ class Transport:
             pass
 
     # -- tables ---------------------------------------------------------------
+    @property
+    def grids(self):
+        """(Moller, outer-shell Gryzinski) TableGrid of a DS configuration, or
+        None; for tables from a batched build, read from the device tables on
+        first use."""
+        lazy = getattr(self, "_lazy_grids", None)
+        if lazy is not None:
+            E0, Emin, Wc = lazy
+            self._lazy_grids = None
+            self._grids = (self._library_grid(_abi.TABLE_MOLLER, moller_grid(self.mat, E0, Emin, Wc)),
+                           self._library_grid(_abi.TABLE_GRYZ, gryz_grid(self.mat, E0, Emin)))
+        return getattr(self, "_grids", None)
+
+    @grids.setter
+    def grids(self, value):
+        self._lazy_grids = None
+        self._grids = value
+
     def _library_grid(self, which, grid):
         ctx = context(self.device)
         nE, nW = C.c_int32(0), C.c_int32(0)
@@ -476,8 +493,8 @@ def grid_search_batch(grids, tol_E, tol_W, device=0, statuses=False):
 
 def build_transports_batch(specs, tol_E=1e-3, tol_W=1e-3, table_precision=_abi.PREC_FP64, device=0, pick_nodes=0):
     """Transports for many configurations with their tables built together
-    (mcd_tables_build_batch: grid searches on host threads, the CDF rows and
-    row records of every table in one device launch each).  specs: list of
+    (mcd_tables_build_batch: the grid searches, the CDF rows and the row
+    records of every table in one device launch each).  specs: list of
     dicts of Transport's keyword arguments (material, E0, Emin, Wc, s_tilt,
     mode, bethe_model, ...); the tables equal those of
     Transport(..., grid_search="library") bit for bit."""
