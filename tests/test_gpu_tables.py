@@ -141,6 +141,33 @@ def test_device_grid_search_equals_host_search():
             assert (int(e), int(w)) == (h.Es.size, h.Ws.size), (g.kind, g.Emax, g.Wmin, tE, tW)
 
 
+@pytest.mark.parametrize("tol", [1e-2, 1e-3, 1e-4])
+def test_device_grid_search_random_configurations(tol):
+    """Device and host searches pick the same sizes (or both fail) for 48
+    random DS configurations per tolerance (seeded: material, E0 in 3-30 keV,
+    Emin from 300 eV to 0.7 E0, Wc in 10-300 eV), 96 tables each."""
+    from mcd.engine import grid_search_batch
+    from mcd.material import Material
+    from mcd.tables import gryz_grid, moller_grid
+    rng = np.random.default_rng(int(-np.log10(tol)) * 7919)
+    mats = {m: Material(m) for m in ("Al", "Si", "Cu", "Au")}
+    grids = []
+    while len(grids) < 96:
+        m = mats[rng.choice(list(mats))]
+        E0 = float(np.round(rng.uniform(3000.0, 30000.0), 1))
+        Emin = float(np.round(max(300.0, E0 * rng.uniform(0.05, 0.7)), 1))
+        Wc = float(np.round(rng.uniform(10.0, 300.0), 2))
+        grids += [moller_grid(m, E0, Emin, Wc), gryz_grid(m, E0, Emin)]
+    nE, nW, st = grid_search_batch(grids, tol, tol, statuses=True)
+    for g, e, w, s_ in zip(grids, nE, nW, st):
+        try:
+            h = g.build_grids_library(tol, tol)
+        except Exception:
+            assert s_ != 0, (g.kind, g.Emax, g.Emin, g.Wmin)
+            continue
+        assert s_ == 0 and (int(e), int(w)) == (h.Es.size, h.Ws.size), (g.kind, g.Emax, g.Emin, g.Wmin, tol)
+
+
 def test_device_grid_search_many_tables_in_chunks():
     """More tables than one launch takes (256 per launch): five copies of the
     batched workload's 64 DS tables give five copies of the 64 tables' sizes."""
