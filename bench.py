@@ -66,6 +66,11 @@ def parse():
                     help="instead of the headline line, run the precision study (FP32 vs the all-FP64 GPU run of "
                          "the same streams; DS Cu 20 keV and continuous Au 30 keV x 3 Bethe laws x 2 tilts) and "
                          "print one JSON line per workload")
+    ap.add_argument("--cli", action="store_true",
+                    help="instead of the headline line, time the doScatter.py command line end to end on the "
+                         "headline configuration (input file, keyed tables, chunked launches, HDF5 store): "
+                         "electron records of 2^22 electrons, then records and four scatter fields of 2^12 "
+                         "electrons, each with the tables built and then read back; one JSON line")
     ap.add_argument("--fixed-shapes", action="store_true",
                     help="skip every autotuner: the headline runs served 1024/8 (unless --block/--guard say "
                          "otherwise), the differentiation forward and the continuous runs the library default, "
@@ -349,6 +354,48 @@ def config_sweep(steps, tune=True):
                           "events_per_s": events / (ms * 1e-3)}), flush=True)
 
 
+def cli_rate():
+    """doScatter.py (the reference's command line, MC/doScatter.py) end to end:
+    parsing the input file, the parameter-keyed tables (built on the first
+    run, read back on the second), chunked launches and the HDF5 store, timed
+    by wall clock on the DS Cu 20 keV 70 deg configuration."""
+    import contextlib
+    import io
+    import re
+    import shutil
+    import tempfile
+
+    import doScatter
+
+    base = ("E0 : 20000\nEmin : 1000\nmaterial : Cu\ns_tilt : 70\nelastic : Ruth_vanilla\nmode : DS\n"
+            "Bethe_model : classical\ndiffMFP : no\ntol_E : 1e-3\ntol_W : 1e-3\nmaxScatt : off\nWc : 50\n"
+            "seed : 1\n")
+    runs = (("els", 1 << 22, "", 1 << 20), ("els+scats", 1 << 12, "type, E_loss, position, direction", 1 << 10))
+    d = tempfile.mkdtemp(prefix="mcd_cli_")
+    out = []
+    try:
+        for gen in ("yes", "no"):
+            for label, n, scat, chunk in runs:
+                f = os.path.join(d, label + ".file")
+                with open(f, "w") as fh:
+                    fh.write(base + f"num_el : {n}\ngen_tables : {gen}\n" +
+                             (f"scatter_output : {scat}\n" if scat else ""))
+                buf = io.StringIO()
+                t = time.perf_counter()
+                with contextlib.redirect_stdout(buf):
+                    path = doScatter.main(["-i", f, "-o", os.path.join(d, "out"), "-c", str(chunk)])
+                wall = time.perf_counter() - t
+                events = int(re.findall(r"(\d+) scatter events", buf.getvalue())[-1])
+                out.append({"output": label, "gen_tables": gen, "electrons": n, "chunk": chunk,
+                            "scatter_output": scat or None, "events": events, "wall_s": wall,
+                            "events_per_s": events / wall, "store_mib": os.path.getsize(path) / 2**20})
+                os.remove(path)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    print(json.dumps({"workload": "doScatter.py end to end, DS Cu 20 keV -> 1 keV, 70 deg, Wc 50 eV, tol 1e-3, "
+                                  "seed 1, HDF5 store", "runs": out}), flush=True)
+
+
 def valu_ceiling(reps=5):
     """Wavefront-instructions per second the SIMDs issue for the VALU
     instructions that dominate the transport's arithmetic and its Philox
@@ -410,6 +457,9 @@ def main():
         return
     if a.valu_ceiling:
         valu_ceiling()
+        return
+    if a.cli:
+        cli_rate()
         return
     if a.precision:
         from mcd import precision

@@ -192,12 +192,26 @@ class H5Store:
         self.frames = {}   # key -> dict(dtype, nrows, spill path, spill file, columns)
         self._closed = False
 
+    @staticmethod
+    def _ascii(a):
+        """A column of str as fixed-width bytes, converted in numpy (None when
+        some string is not ASCII: the caller then encodes one by one)."""
+        try:
+            return np.asarray(a).astype("S")
+        except (UnicodeEncodeError, ValueError, TypeError):
+            return None
+
     def _dtype_for(self, key, cols, min_itemsize):
         fields = [("index", "<i8")]
         for name, v in cols.items():
             a = np.asarray(v)
             if a.dtype.kind in "USO":
-                w = max([len(str(x).encode()) for x in a.tolist()] + [1, int((min_itemsize or {}).get(name, 0))])
+                b = self._ascii(a) if a.dtype.kind != "S" else a
+                if b is not None:
+                    longest = int(np.strings.str_len(b).max()) if b.size else 0
+                else:
+                    longest = max([len(str(x).encode()) for x in a.tolist()] + [0])
+                w = max(longest, 1, int((min_itemsize or {}).get(name, 0)))
                 fields.append((name, f"S{w}"))
             elif a.dtype.kind == "b":
                 fields.append((name, "<i1"))
@@ -225,15 +239,19 @@ class H5Store:
         for name in f["columns"]:
             a = np.asarray(columns[name])
             if dt[name].kind == "S":
-                enc = np.array([str(x).encode() for x in a.tolist()], dtype=object)
-                width = max((len(x) for x in enc), default=0)
+                b = self._ascii(a) if a.dtype.kind != "S" else a
+                if b is not None:
+                    width = int(np.strings.str_len(b).max()) if b.size else 0
+                else:
+                    b = np.array([str(x).encode() for x in a.tolist()], dtype=object)
+                    width = max((len(x) for x in b), default=0)
                 if width > dt[name].itemsize:
                     raise ValueError(f"{key}.{name}: string of {width} bytes above the column width "
                                      f"{dt[name].itemsize} (raise min_itemsize)")
-                rec[name] = enc.astype(dt[name])
+                rec[name] = b.astype(dt[name])
             else:
                 rec[name] = a
-        f["fh"].write(rec.tobytes())
+        rec.tofile(f["fh"])
         f["nrows"] += n
 
     def put_input(self, params: dict):
@@ -485,6 +503,15 @@ def read_table(path, key):
     return arr, tattrs, r.attrs(gm)
 
 
+def _strings(a):
+    """A fixed-width bytes column as str: numpy's conversion for ASCII, UTF-8
+    (the store's encoding attribute) one by one otherwise."""
+    try:
+        return a.astype(str)
+    except UnicodeDecodeError:
+        return np.array([x.decode("utf-8") for x in a.tolist()], dtype=object)
+
+
 def read_frame(path, key):
     """The frame stored under key as a pandas DataFrame (byte-string columns
     decoded to str), indexed by the stored index column."""
@@ -494,7 +521,7 @@ def read_frame(path, key):
     cols = {}
     for name in arr.dtype.names[1:]:
         a = arr[name]
-        cols[name] = a.astype(str) if a.dtype.kind == "S" else a
+        cols[name] = _strings(a) if a.dtype.kind == "S" else a
     return pd.DataFrame(cols, index=pd.Index(arr["index"], name=None))
 
 

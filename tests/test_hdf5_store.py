@@ -76,6 +76,24 @@ def test_round_trip(tmp_path):
     assert not glob.glob(path + ".*.part")
 
 
+def test_string_columns_ascii_and_not(tmp_path):
+    """String columns are stored as fixed-width bytes: ASCII text (object or
+    unicode arrays) converted in numpy, other text encoded one by one as UTF-8
+    (the width then counts bytes); both read back as the same strings."""
+    path = str(tmp_path / "s.h5")
+    names = np.array(["Ruth", "Moller", "Gryz3p"], dtype=object)
+    with hdf5.H5Store(path) as st:
+        st.append("a", {"type": names[[0, 1, 2, 1]], "u": np.array(["x", "yz", "", "w"])})
+        st.append("b", {"type": np.array(["Møller", "Ruth"], dtype=object)})
+    arr, _, _ = hdf5.read_table(path, "a")
+    assert arr.dtype["type"].itemsize == 6 and arr.dtype["u"].itemsize == 2
+    assert list(hdf5.read_frame(path, "a")["type"]) == ["Ruth", "Moller", "Gryz3p", "Moller"]
+    assert list(hdf5.read_frame(path, "a")["u"]) == ["x", "yz", "", "w"]
+    arr, _, _ = hdf5.read_table(path, "b")
+    assert arr.dtype["type"].itemsize == len("Møller".encode())
+    assert list(hdf5.read_frame(path, "b")["type"]) == ["Møller", "Ruth"]
+
+
 def test_column_width_and_schema_are_enforced(tmp_path):
     with pytest.raises(ValueError, match="min_itemsize"):
         with hdf5.H5Store(str(tmp_path / "a.h5")) as st:
