@@ -32,6 +32,11 @@ constexpr int64_t kMaxStepsOff = 1ll << 24;  // step cap when maxScatt is off
 // read by the 64 lanes of a wavefront (one node each) start in different LDS
 // banks.
 __host__ __device__ constexpr int pick_stride(int n_ch) { return (n_ch + 2) | 1; }
+// The wide node record of an LDS copy: the n_ch + 2 floats of the table's
+// record, then the differences to the next node of its n_ch - 1 cumulative
+// probabilities and of its log2 mean free path (the interpolation's slopes,
+// formed once when the copy is staged instead of at every step).
+__host__ __device__ constexpr int pick_stride_wide(int n_ch) { return (2 * n_ch + 2) | 1; }
 
 constexpr uint32_t kMetaNoSwap = 15u;
 constexpr uint32_t kMetaExact = 1u << 16;
@@ -327,25 +332,26 @@ struct PickEval {
   float mfp_ln2;         // mean free path times ln 2: the step is -mfp_ln2 log2(u)
 };
 
-template <int NCH>
+template <int NCH, bool WIDE = false>
 __device__ __forceinline__ void f_pick_eval(const float* __restrict__ nodes, const DevPick& P, float E,
                                             PickEval<NCH>& o) {
   constexpr int NB = NCH - 1;
-  constexpr int S = pick_stride(NCH);
+  constexpr int S = WIDE ? pick_stride_wide(NCH) : pick_stride(NCH);
+  constexpr int D = NB + 3;  // WIDE: the slopes follow the record
   float x = (log2_f(E) - P.log2E0) * P.inv_dlog2E;
   x = fminf(fmaxf(x, 0.f), (float)(P.n_nodes - 1));
   const int k = min((int)x, P.n_nodes - 2);
   const float f = x - (float)k;
   const float* a = nodes + k * S;
   const float* b = a + S;
-  o.mfp_ln2 = exp2_f(a[NB] + f * (b[NB] - a[NB]));
+  o.mfp_ln2 = exp2_f(a[NB] + f * (WIDE ? a[D + NB] : b[NB] - a[NB]));
   o.perm_lo = __float_as_uint(a[NB + 1]);
   const uint32_t meta = __float_as_uint(a[NB + 2]);
   o.perm_hi = meta & 0xFFFu;
   o.exact = (meta & kMetaExact) != 0;
   o.swap = (int)((meta >> 12) & 15u);
 #pragma unroll
-  for (int i = 0; i < NB; ++i) o.B[i] = a[i] + f * (b[i] - a[i]);
+  for (int i = 0; i < NB; ++i) o.B[i] = a[i] + f * (WIDE ? a[D + i] : b[i] - a[i]);
   o.pA = o.pB = 0.f;
   if (o.swap != (int)kMetaNoSwap) {
     // node k: channel A at position j, B at j+1; node k+1: B at j, A at j+1
@@ -380,10 +386,10 @@ __device__ __forceinline__ int perm_label(uint32_t lo, uint32_t hi, int pos) {
 
 // Returns the channel label, or -1 when the guard band (or an exact-only cell)
 // requires the FP64 pick.  mfp_ln2 is always set.
-template <int NCH>
+template <int NCH, bool WIDE = false>
 __device__ __forceinline__ int f_pick_table(const float* __restrict__ nodes, const DevPick& P, float E, float u,
                                             float& mfp_ln2, PickEval<NCH>& o) {
-  f_pick_eval<NCH>(nodes, P, E, o);
+  f_pick_eval<NCH, WIDE>(nodes, P, E, o);
   mfp_ln2 = o.mfp_ln2;
   if (o.exact) return -1;
   if (o.swap != (int)kMetaNoSwap && fabsf(o.pA - o.pB) <= P.delta) {

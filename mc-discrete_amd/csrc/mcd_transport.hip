@@ -34,6 +34,29 @@ __device__ __forceinline__ DevTable* stage_single(const DevCfg& c, float* lds) {
   return tabs;
 }
 
+// The served kernel's LDS copy of the pick table in the wide record
+// (pick_stride_wide): each node's record, then its differences to the next
+// node, each the same float subtraction the compact record's interpolation
+// does at every step, so the interpolated values are the same bit for bit.
+template <int NCH>
+__device__ __forceinline__ DevTable* stage_single_wide(const DevCfg& c, float* lds) {
+  constexpr int S = pick_stride(NCH), SW = pick_stride_wide(NCH), NB = NCH - 1, D = NB + 3;
+  const int n = c.pick.n_nodes;
+  const float* __restrict__ g = c.pick.nodes;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const float* a = g + (size_t)k * S;
+    float* o = lds + (size_t)k * SW;
+    for (int i = 0; i < D; ++i) o[i] = a[i];
+    for (int i = 0; i <= NB; ++i) o[D + i] = k + 1 < n ? a[S + i] - a[i] : 0.f;
+  }
+  DevTable* tabs = reinterpret_cast<DevTable*>(lds + tab_offset_floats(n * SW));
+  if (threadIdx.x == 0) {
+    tabs[0] = c.tm;
+    tabs[1] = c.tg;
+  }
+  return tabs;
+}
+
 // The record is written once, when the trajectory ends, as nine dword
 // stores marked non-temporal: records of one wavefront land in different
 // cache lines at different times, and streaming them past L2 keeps them from
@@ -215,7 +238,7 @@ __device__ void guard_service(const DevCfg& c, GuardBox* boxes, uint32_t* pendin
 // Philox counter.  PHASE 3 (DS): guard-band picks go to the workgroup's
 // service wavefront through `box`; same records as phase 0.
 // EXPL (continuous mode): the explicit Bethe law, evaluated in FP64.
-template <int MODE, int NCH, int PHASE = 0, bool PERIOD = false, bool EXPL = false>
+template <int MODE, int NCH, int PHASE = 0, bool PERIOD = false, bool EXPL = false, bool WIDE = false>
 __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __restrict__ lds,
                                                const DevTable* __restrict__ tabs, const RunArgs& R,
                                                uint32_t& n_events, uint32_t& n_fp64, GuardBox* box = nullptr,
@@ -310,7 +333,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
       u_az = u01f(r.w);
       float mfp_ln2;
       PickEval<NCH> pe;
-      ch = f_pick_table<NCH>(lds, c.pick, E.hi, u0, mfp_ln2, pe);
+      ch = f_pick_table<NCH, WIDE>(lds, c.pick, E.hi, u0, mfp_ln2, pe);
       if constexpr (PHASE == 1) {
         if (ch < 0) {  // guard band: park the electron for the resume kernel
           const uint64_t m = __ballot(true);
@@ -503,12 +526,13 @@ __global__ __launch_bounds__(256, MINW) void k_transport_f32(DevCfg c, RunArgs R
   wave_add_counters(R.counters + MCD_CNT_FP64_PICKS, cnt + 1, 1, lane);
 }
 
-// Service-guard kernel (DS, FP32).  LDS: the pick table, the two loss-table
+// Service-guard kernel (DS, FP32).  LDS: the pick table (wide records), the two loss-table
 // descriptors, one GuardBox per transport wavefront, the finished-wavefront
 // count and the pending-mailbox bits.  Wavefront 0 serves the guard picks, the others run the transport.
 // MINW: resident wavefronts per SIMD the register allocation must allow.
 __host__ __device__ inline size_t svc_box_offset(const DevCfg& c) {
-  return ((size_t)tab_offset_floats(c.pick.n_nodes * c.pick.stride) * sizeof(float) + 2 * sizeof(DevTable) + 15) &
+  return ((size_t)tab_offset_floats(c.pick.n_nodes * pick_stride_wide(c.pick.n_ch)) * sizeof(float) +
+          2 * sizeof(DevTable) + 15) &
          ~(size_t)15;
 }
 static size_t svc_lds_bytes(const DevCfg& c, int block) {
@@ -518,7 +542,7 @@ static size_t svc_lds_bytes(const DevCfg& c, int block) {
 template <int NCH, int MINW>
 __global__ __launch_bounds__(1024, MINW) void k_transport_svc(DevCfg c, RunArgs R) {
   extern __shared__ float lds[];
-  DevTable* tabs = stage_single(c, lds);
+  DevTable* tabs = stage_single_wide<NCH>(c, lds);
   const int n_boxes = (int)(blockDim.x >> 6) - 1;
   GuardBox* boxes = reinterpret_cast<GuardBox*>(reinterpret_cast<char*>(lds) + svc_box_offset(c));
   uint32_t* done = reinterpret_cast<uint32_t*>(boxes + n_boxes);
@@ -538,7 +562,8 @@ __global__ __launch_bounds__(1024, MINW) void k_transport_svc(DevCfg c, RunArgs 
     return;
   }
   uint32_t n_events = 0, n_fp64 = 0;
-  transport_loop<MCD_MODE_DS, NCH, 3, true>(c, lds, tabs, R, n_events, n_fp64, boxes + (w - 1), pending);
+  transport_loop<MCD_MODE_DS, NCH, 3, true, false, true>(c, lds, tabs, R, n_events, n_fp64, boxes + (w - 1),
+                                                         pending);
   const int lane = threadIdx.x & 63;
   // every lane is converged here; the service wavefront stops once all are counted
   if (lane == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
