@@ -392,9 +392,12 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
         }
       }
       l = -mfp_ln2 * log2_f(u1);
+      // 1/E for both the screening parameter and the loss fraction: one
+      // reciprocal for the wavefront instead of one in each channel branch
+      const float rE = __builtin_amdgcn_rcpf(E.hi);
       if (ch == MCD_CH_RUTH) {  // Rutherford_halfPol, scattering.py:125-152
         Wh = 0.f;
-        const float a = c.c_alpha_f * __builtin_amdgcn_rcpf(E.hi);
+        const float a = c.c_alpha_f * rE;
         const float om = 1.f - u2;
         const float inv = __builtin_amdgcn_rcpf(om + a);
         s2 = a * u2 * inv;
@@ -414,7 +417,7 @@ __device__ __forceinline__ void transport_loop(const DevCfg& c, const float* __r
         const float w2 = 2.f * Wh;
         const bool ok = !(D.hi < w2 || (D.hi == w2 && D.lo < 0.f));
         if (ok) {
-          const float x = Wh * __builtin_amdgcn_rcpf(E.hi);
+          const float x = Wh * rE;
           const float sq = __builtin_amdgcn_sqrtf(1.f - x);
           c2 = 0.5f * (1.f + sq);
           s2 = 0.5f * x * __builtin_amdgcn_rcpf(1.f + sq);
