@@ -317,7 +317,8 @@ def config_sweep(steps, tune=True):
     kernel on Si 10 keV (2,000 electrons, seed 0), the DS headline (Cu 20 keV,
     70 deg, 2^22 electrons, seed 1) and continuous Au 30 keV -> 1 keV with the
     classical, JL and explicit Bethe laws at 0 and 70 deg (2^22 electrons,
-    seed 2), each at its autotuned launch shape (FP64: library default)."""
+    seed 2), and the DS headline with each of the other elastic models, each
+    at its autotuned launch shape (FP64: library default)."""
     import torch
 
     from mcd import _abi, engine
@@ -331,6 +332,10 @@ def config_sweep(steps, tune=True):
             runs.append((f"continuous {law} Au 30 keV -> 1 keV, {tilt:g} deg", 1 << 22, 2,
                          dict(material="Au", E0=30000.0, Emin=1000.0, s_tilt=tilt, mode="cont", bethe_model=law),
                          False))
+    # the other elastic models (crossSections.py:63-147) on the DS headline
+    for model in ("Ruth_vanilla_wDefl", "Ruth_nigram", "Ruth_nigram_wDefl"):
+        runs.append((f"DS Cu 20 keV -> 1 keV, 70 deg, Wc 50 eV, tol 1e-3, elastic {model}", 1 << 22, 1,
+                     dict(material="Cu", E0=20000.0, Emin=1000.0, Wc=50.0, s_tilt=70.0, elastic=model), False))
     for name, n, seed, kw, fp64 in runs:
         tr = engine.Transport(**kw)
         prec = _abi.PREC_FP64 if fp64 else _abi.PREC_FP32
