@@ -687,10 +687,9 @@ extern "C" int mcd_grid_search_batch(mcd_ctx* ctx, const mcd_grid_spec* specs, i
     if (status) status[i] = res[i].status ? MCD_ERR_TABLE : MCD_OK;
     if (res[i].status && first_bad < 0) first_bad = i;
   }
-  if (first_bad >= 0) {
-    set_err(ctx, MCD_ERR_TABLE, "table %d: %s", first_bad, grid_status_text(res[first_bad].status));
-    if (!status) return MCD_ERR_TABLE;
-  }
+  // with a status array the per-table codes are the report and the call succeeds
+  if (first_bad >= 0 && !status)
+    return set_err(ctx, MCD_ERR_TABLE, "table %d: %s", first_bad, grid_status_text(res[first_bad].status));
   return MCD_OK;
 }
 
