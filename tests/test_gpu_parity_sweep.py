@@ -10,7 +10,8 @@ host-only mcd_grid_search, so drawing needs no device.  For each: the FP64
 kernel equals the oracle
 trajectory by trajectory (same Philox stream, same table bytes), and the
 FP32 kernel has >= 99.9 % of its trajectories identical and the rest's
-energies and directions within the FP32 tolerances of conftest.check_values."""
+energies and directions within the FP32 tolerances of conftest.check_values;
+for DS, the served-guard form gives the FP32 records bit for bit."""
 import numpy as np
 import pytest
 
@@ -95,3 +96,8 @@ def test_configuration_matches_oracle(c):
     assert same.mean() >= 0.999, same.mean()
     check_values(r.els, o, same)
     assert int(r.counters[_abi.CNT_ELECTRONS]) == n
+    if c["mode"] == "DS":
+        # the served form (the kernel large launches run: wide pick records in
+        # LDS, FP64 picks by a service wavefront) gives the same records
+        rs = tr.run(n, seed=c["seed"], guard="service")
+        assert rs.els.tobytes() == r.els.tobytes()
